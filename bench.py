@@ -1,0 +1,181 @@
+"""Benchmark of the DGSEM RHS + LSERK hot path on one MI355X.
+
+Default workload (the headline configuration): Taylor-Green vortex Re=1600, Ma=0.1 on [0,2pi]^3, 32^3
+elements, N=7 Gauss-Lobatto (16.8M DOF), split-form Pirozzoli volume flux, Lax-Friedrichs surface flux, BR1
+lifting, Carpenter 5-stage LSERK, FP64, synthetic initial state with seeded 1e-3 velocity noise, dt from
+CALCTIMESTEP at t=0 held fixed.  One "step" = one LSERK step (5 RHS stages) over the whole mesh.
+value = nDOF * 5 * steps / wall seconds (DOF-updates/s), inputs resident in HBM before the timed region.
+
+Also reported (extra keys): PID, the same configuration at 48^3 in FP64 and in the mixed form with their
+ratio, per-kernel times, the roofline of the dominant kernel and the oracle's CPU throughput.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PID & DOF-updates/s, FP64 vs mixed at stated L2; adjoint/forward cost ratio"
+HBM_PEAK_GBS = 8000.0
+NSTAGES = 5
+
+# algorithmic HBM bytes per DOF-stage for the two passes of one stage (3-pass BR1 model with the trace pass
+# fused into the update pass): state b_s=8, geometry b_g=8, traces b_t (8 FP64 / 4 mixed), phi = 6/(N+1)
+def alg_bytes(N, b_t):
+    phi = 6.0 / (N + 1)
+    b_s = b_g = 8.0
+    lift = 5 * b_s + 10 * b_g + phi * 5 * b_t + phi * 4 * b_g + phi * 15 * b_t
+    rhs = 5 * b_s + 10 * b_g + phi * 10 * b_g + phi * 5 * b_t + phi * 15 * b_t + 3 * 5 * 8 + phi * 5 * b_t
+    return lift, rhs
+
+
+def tgv_case(ne, N=7, noise=1e-3):
+    from galaexi_amd import mesh as M
+    from galaexi_amd import testcases as T
+
+    m = M.box_mesh((ne, ne, ne), N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3)
+    U = T.taylor_green(m.Elem_xGP, Ma=0.1, noise=noise, seed=1600)
+    return m, U
+
+
+# CFL 0.5: with 1e-3 i.i.d. nodal velocity noise the split-PI/LF scheme is unstable at the tutorial CFL 0.9 (the
+# FP64 oracle blows up the same way); 0.5 keeps 1000+ steps stable.
+TGV_GAS = dict(kappa=1.4, R=71.42857, Pr=0.72, mu0=1.0 / 1600.0, CFLScale=0.5, DFLScale=0.5)
+
+
+def run_gpu(m, U, precision, steps, warmup, dt=None):
+    from galaexi_amd import dg
+
+    op = dg.DGOperator(m.N, form="split", riemann="lf", precision=precision, **TGV_GAS)
+    op.set_mesh(m)
+    op.set_state(U)
+    if dt is None:
+        dt, err = op.CalcTimeStep()
+        assert err == 0, err
+    t = op.TimeStepByLSERKW2(0.0, dt, nSteps=warmup)
+    op.timers(True)
+    op.timers_reset()
+    t0 = time.perf_counter()
+    op.TimeStepByLSERKW2(t, dt, nSteps=steps)  # one call: no host synchronisation between steps
+    t1 = time.perf_counter()
+    ker = {k: op.timer(k) for k in ("lift", "rhs")}
+    op.timers(False)
+    Uend = op.get_state()
+    mem = op.memory_bytes()
+    op.close()
+    return dict(wall=t1 - t0, kernels=ker, dt=dt, U=Uend, mem=mem)
+
+
+def cpu_baseline(N=7, ne=16):
+    """The FP64 oracle (C/OpenMP restatement of the reference operator) on all host cores, one LSERK step of
+    the same TGV configuration on a 16^3 mesh (same per-DOF work as the headline mesh)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    m, U = tgv_case(ne, N)
+    cores = os.cpu_count() or 1
+    O.set_threads(cores)
+    op = O.Operator(m, split=True, riemann=O.RIEMANN_LF, kappa=1.4, R=71.42857, Pr=0.72, mu0=1.0 / 1600.0)
+    dt, _e, _c, _v = op.calc_dt(U, TGV_GAS["CFLScale"], TGV_GAS["DFLScale"])
+    K = np.zeros_like(U, order="F")
+    Uc = np.asfortranarray(U.copy())
+    op.rhs(Uc)  # first touch
+    t0 = time.perf_counter()
+    op.lserk_step(Uc, K, 0.0, dt)
+    t1 = time.perf_counter()
+    return dict(value=m.nDOF * NSTAGES / (t1 - t0), unit="DOF-updates/s", cores=cores, kind="port",
+                sample=f"oracle FP64, one LSERK step (5 stages) of TGV N={N} {ne}^3 ({m.nDOF} DOF), "
+                       f"{t1 - t0:.2f} s wall on {cores} threads",
+                cpu=platform.processor() or platform.machine())
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ne", type=int, default=32, help="elements per direction of the headline workload")
+    ap.add_argument("--compare-ne", type=int, default=48, help="mesh for the FP64-vs-mixed comparison (0: skip)")
+    ap.add_argument("--compare-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one rank's element share per GPU; "
+              "run with --gpus 1)", file=sys.stderr)
+        return 2
+    N = 7
+    m, U = tgv_case(a.ne, N)
+    print(f"# headline: TGV N={N} {a.ne}^3 ({m.nDOF} DOF) FP64, {a.steps} steps", file=sys.stderr, flush=True)
+    r = run_gpu(m, U, "fp64", a.steps, a.warmup)
+    nDOF = m.nDOF
+    value = nDOF * NSTAGES * a.steps / r["wall"]
+    ms_step = 1e3 * r["wall"] / a.steps
+    lift_b, rhs_b = alg_bytes(N, 8)
+    ms_rhs, cnt_rhs = r["kernels"]["rhs"]
+    ms_lift, cnt_lift = r["kernels"]["lift"]
+    rhs_avg_s = ms_rhs / cnt_rhs / 1e3
+    lift_avg_s = ms_lift / cnt_lift / 1e3
+    achieved = rhs_b * nDOF / rhs_avg_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (TGV initial state + seeded 1e-3 velocity noise, analytic periodic box mesh)",
+        "config": {"workload": f"TGV Re1600 Ma0.1 [0,2pi]^3 {a.ne}^3 elems N={N} GL, split-PI + LF, BR1, "
+                               f"Carpenter LSERK4-5, FP64", "nDOF": nDOF, "N": N, "elements": m.nElems,
+                   "dt": r["dt"]},
+        "pid_s": 1.0 / value,
+        "kernels_ms_avg": {"lift": 1e3 * lift_avg_s, "rhs": 1e3 * rhs_avg_s},
+        "roofline": {"bound": "hbm", "kernel": "k_rhs<7,double>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_dof": rhs_b},
+        "stage_roofline": {"alg_bytes_per_dof_stage": lift_b + rhs_b,
+                           "achieved_GBs": (lift_b + rhs_b) * nDOF / (lift_avg_s + rhs_avg_s) / 1e9},
+        "device_memory_GB": r["mem"] / 1e9,
+    }
+    del m, U, r
+    if a.compare_ne:
+        m2, U2 = tgv_case(a.compare_ne, N)
+        print(f"# compare: TGV N={N} {a.compare_ne}^3 ({m2.nDOF} DOF) FP64 vs mixed, {a.compare_steps} steps",
+              file=sys.stderr, flush=True)
+        r64 = run_gpu(m2, U2, "fp64", a.compare_steps, 1)
+        r32 = run_gpu(m2, U2, "mixed", a.compare_steps, 1, dt=r64["dt"])
+        v64 = m2.nDOF * NSTAGES * a.compare_steps / r64["wall"]
+        v32 = m2.nDOF * NSTAGES * a.compare_steps / r32["wall"]
+        Ua, Ub = r64["U"], r32["U"]
+        l2 = [float(np.linalg.norm((Ub[v] - Ua[v]).ravel()) / np.linalg.norm(Ua[v].ravel())) for v in range(5)]
+        out["compare"] = {"workload": f"TGV N={N} {a.compare_ne}^3 ({m2.nDOF} DOF), {a.compare_steps}+1 steps",
+                          "fp64_dof_updates_per_s": v64, "mixed_dof_updates_per_s": v32,
+                          "mixed_over_fp64": v32 / v64,
+                          "mixed_rel_l2_per_var_after_steps": l2,
+                          "fp64_kernels_ms_avg": {k: r64["kernels"][k][0] / r64["kernels"][k][1]
+                                                  for k in ("lift", "rhs")},
+                          "mixed_kernels_ms_avg": {k: r32["kernels"][k][0] / r32["kernels"][k][1]
+                                                   for k in ("lift", "rhs")}}
+        del m2, U2, r64, r32
+    if not a.no_cpu_baseline:
+        print("# cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

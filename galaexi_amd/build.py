@@ -1,0 +1,72 @@
+"""Build libgalx.so (gfx950) in-tree with hipcc.
+
+python -m galaexi_amd.build [--degrees 3,5,7] [--mixed-degrees 3,5,7]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libgalx.so")
+
+DEFAULT_FP64 = (1, 2, 3, 4, 5, 6, 7, 8)
+DEFAULT_MIXED = (3, 5, 7)
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    # no packed FP32 instructions (v_pk_*_f32) anywhere in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def sources():
+    return [os.path.join(CSRC, "galx_api.hip")]
+
+
+def deps():
+    return [os.path.join(CSRC, f) for f in os.listdir(CSRC)] + [os.path.join(HERE, "..", "include", "galx.h")]
+
+
+def build(degrees=DEFAULT_FP64, mixed_degrees=DEFAULT_MIXED, out=LIB, verbose=False, resource_report=None):
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    xs = " ".join(f"X({d})" for d in degrees)
+    xm = " ".join(f"X({d})" for d in mixed_degrees)
+    cmd = [hipcc] + HIPCC_FLAGS + [f"-DGALX_DEGREES_FP64={xs}", f"-DGALX_DEGREES_MIXED={xm}"]
+    if resource_report:
+        cmd.append("-Rpass-analysis=kernel-resource-usage")
+    tmp = out + ".tmp"
+    cmd += ["-o", tmp] + sources()
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if resource_report:
+        with open(resource_report, "w") as f:
+            f.write(res.stderr)
+    if res.returncode != 0:
+        sys.stderr.write(res.stderr[-20000:])
+        raise RuntimeError("hipcc failed")
+    os.replace(tmp, out)
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--degrees", default=",".join(map(str, DEFAULT_FP64)))
+    ap.add_argument("--mixed-degrees", default=",".join(map(str, DEFAULT_MIXED)))
+    ap.add_argument("--report", default=None, help="write the kernel resource-usage remarks to this file")
+    a = ap.parse_args(argv)
+    degs = tuple(int(x) for x in a.degrees.split(",") if x)
+    mdegs = tuple(int(x) for x in a.mixed_degrees.split(",") if x)
+    print(build(degs, mdegs, verbose=True, resource_report=a.report))
+
+
+if __name__ == "__main__":
+    main()

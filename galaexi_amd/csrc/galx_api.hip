@@ -1,0 +1,938 @@
+// galx_api.hip -- C ABI (include/galx.h) of the gfx950 DGSEM engine: context, mesh ingest, state transfer,
+// stage orchestration and the reference-layout unit entry points.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/galx.h"
+#include "galx_kernels.hpp"
+
+using namespace galx;
+
+// ----------------------------------------------------------------------------------------------------------
+// errors
+// ----------------------------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                          \
+    do {                                                                                                   \
+        hipError_t _e = (x);                                                                               \
+        if (_e != hipSuccess) return fail(GALX_E_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(_e), \
+                                          __FILE__, __LINE__);                                              \
+    } while (0)
+
+// ----------------------------------------------------------------------------------------------------------
+// 1D Gauss-Lobatto basis (Legendre recurrences, Newton on L_{N+1}-L_{N-1}); InitDGbasis (src/dg/dg.f90:181-242)
+// ----------------------------------------------------------------------------------------------------------
+static void lgl_q(int N, double x, double& q, double& qd, double& L)
+{
+    double a = 1.0, b = x, l = x;
+    for (int m = 2; m <= N; m++) {
+        l = ((2.0 * m - 1.0) * x * b - (m - 1.0) * a) / m;
+        a = b;
+        b = l;
+    }
+    q = (2.0 * N + 1.0) / (N + 1.0) * (x * l - a);
+    qd = (2.0 * N + 1.0) * l;
+    L = l;
+}
+
+struct Basis {
+    int N;
+    std::vector<double> x, w, D, DT, DhT, DVS;  // D(i,l) at [i + n*l]
+};
+
+static Basis make_basis(int N)
+{
+    Basis B;
+    B.N = N;
+    int n = N + 1;
+    B.x.assign(n, 0.0);
+    B.w.assign(n, 0.0);
+    B.x[0] = -1.0;
+    B.x[N] = 1.0;
+    B.w[0] = B.w[N] = 2.0 / (N * (N + 1.0));
+    for (int m = 1; m <= (N + 1) / 2 - 1; m++) {
+        double xm = -cos(M_PI / N * (m + 0.25) - 3.0 / (8.0 * N * M_PI) / (m + 0.25));
+        double q, qd, L;
+        for (int it = 0; it <= 10; it++) {
+            lgl_q(N, xm, q, qd, L);
+            double dx = -q / qd;
+            xm += dx;
+            if (fabs(dx) < 1e-15 * fabs(xm)) break;
+        }
+        lgl_q(N, xm, q, qd, L);
+        B.x[m] = xm;
+        B.x[N - m] = -xm;
+        B.w[m] = B.w[N - m] = B.w[0] / (L * L);
+    }
+    if (N % 2 == 0) {
+        double q, qd, L;
+        lgl_q(N, 0.0, q, qd, L);
+        B.x[N / 2] = 0.0;
+        B.w[N / 2] = B.w[0] / (L * L);
+    }
+    std::vector<double> wb(n, 1.0);
+    for (int a = 1; a < n; a++)
+        for (int b = 0; b < a; b++) {
+            wb[b] *= B.x[b] - B.x[a];
+            wb[a] *= B.x[a] - B.x[b];
+        }
+    for (auto& v : wb) v = 1.0 / v;
+    B.D.assign(n * n, 0.0);
+    for (int l = 0; l < n; l++)
+        for (int i = 0; i < n; i++)
+            if (l != i) {
+                B.D[i + n * l] = wb[l] / (wb[i] * (B.x[i] - B.x[l]));
+                B.D[i + n * i] -= B.D[i + n * l];
+            }
+    B.DT.assign(n * n, 0.0);
+    B.DhT.assign(n * n, 0.0);
+    B.DVS.assign(n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            B.DT[i + n * j] = B.D[j + n * i];
+            B.DhT[j + n * i] = -((1.0 / B.w[i]) * (B.D[j + n * i] * B.w[j]));  // D_Hat(i,j) stored transposed
+            B.DVS[i + n * j] = B.D[j + n * i];
+        }
+    B.DVS[0] += 1.0 / (2.0 * B.w[0]);
+    B.DVS[N + n * N] -= 1.0 / (2.0 * B.w[N]);
+    return B;
+}
+
+// ----------------------------------------------------------------------------------------------------------
+// layout transposes: reference U(5,n3,nElems) <-> element planes [e][5][n3]
+// ----------------------------------------------------------------------------------------------------------
+__global__ void k_ref_to_planes(long nDOF, int n3, const double* __restrict__ src, double* __restrict__ dst)
+{
+    long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nDOF) return;
+    long e = x / n3;
+    int node = (int)(x - e * n3);
+#pragma unroll
+    for (int v = 0; v < 5; v++) dst[(e * 5 + v) * n3 + node] = src[5 * x + v];
+}
+
+__global__ void k_planes_to_ref(long nDOF, int n3, const double* __restrict__ src, double* __restrict__ dst)
+{
+    long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nDOF) return;
+    long e = x / n3;
+    int node = (int)(x - e * n3);
+#pragma unroll
+    for (int v = 0; v < 5; v++) dst[5 * x + v] = src[(e * 5 + v) * n3 + node];
+}
+
+// ----------------------------------------------------------------------------------------------------------
+// kernel sets per (N, precision, form)
+// ----------------------------------------------------------------------------------------------------------
+struct GasD {
+    double kappa, R, mu, lambda;
+};
+
+struct KernelSet {
+    bool ok = false;
+    int block = 0;
+    void (*trace)(hipStream_t, int, const double*, Mesh, void*) = nullptr;
+    void (*lift)(hipStream_t, int, const double*, Mesh, const void*, void*, const void*, GasD, double, double) = nullptr;
+    void (*rhs)(hipStream_t, int, double*, Mesh, const void*, const void*, void*, double*, double*, const void*, GasD,
+                int, double, double, Stage) = nullptr;
+    void (*dt)(hipStream_t, int, const double*, const double*, double, double, double, double, double, double*,
+               double*, int*) = nullptr;
+};
+
+template <typename T>
+static Gas<T> to_gas(GasD g)
+{
+    return Gas<T>{T(g.kappa), T(g.R), T(g.mu), T(g.lambda)};
+}
+
+template <int N, typename T, bool SPLIT>
+struct Launch {
+    using D = Dm<N>;
+    static void trace(hipStream_t s, int nE, const double* U, Mesh m, void* utr)
+    {
+        hipLaunchKernelGGL((k_trace<N, T>), dim3(nE), dim3(D::block), 0, s, U, m, (T*)utr);
+    }
+    static void lift(hipStream_t s, int nE, const double* U, Mesh m, const void* utr, void* gtr, const void* mats,
+                     GasD g, double LHm, double LHp)
+    {
+        hipLaunchKernelGGL((k_lift<N, T, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr, (T*)gtr,
+                           (const T*)mats, to_gas<T>(g), T(LHm), T(LHp));
+    }
+    static void rhs(hipStream_t s, int nE, double* U, Mesh m, const void* utr, const void* gtr, void* utr_next,
+                    double* Ut, double* K, const void* mats, GasD g, int riemann, double LHm, double LHp, Stage st)
+    {
+        hipLaunchKernelGGL((k_rhs<N, T, T, SPLIT>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr,
+                           (const T*)gtr, (T*)utr_next, Ut, K, (const T*)mats, to_gas<T>(g), riemann, T(LHm), T(LHp),
+                           st);
+    }
+    static void dt(hipStream_t s, int nE, const double* U, const double* geo, double kappa, double mu0, double KPmax,
+                   double CFL, double DFL, double* dtc, double* dtv, int* err)
+    {
+        hipLaunchKernelGGL((k_dt_elem<N>), dim3(nE), dim3(D::block), 0, s, U, geo, kappa, mu0, KPmax, CFL, DFL, dtc,
+                           dtv, err);
+    }
+    static KernelSet make()
+    {
+        KernelSet k;
+        k.ok = true;
+        k.block = D::block;
+        k.trace = &trace;
+        k.lift = &lift;
+        k.rhs = &rhs;
+        k.dt = &dt;
+        return k;
+    }
+};
+
+#ifndef GALX_DEGREES_FP64
+#define GALX_DEGREES_FP64 X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
+#endif
+#ifndef GALX_DEGREES_MIXED
+#define GALX_DEGREES_MIXED X(3) X(5) X(7)
+#endif
+
+static KernelSet select_kernels(int N, int prec, int form)
+{
+    bool split = form == GALX_FORM_SPLIT_PI;
+    if (prec == GALX_PREC_FP64) {
+#define X(NN)                                                                                            \
+    if (N == NN) return split ? Launch<NN, double, true>::make() : Launch<NN, double, false>::make();
+        GALX_DEGREES_FP64
+#undef X
+    } else {
+#define X(NN)                                                                                          \
+    if (N == NN) return split ? Launch<NN, float, true>::make() : Launch<NN, float, false>::make();
+        GALX_DEGREES_MIXED
+#undef X
+    }
+    return KernelSet{};
+}
+
+// ----------------------------------------------------------------------------------------------------------
+// context
+// ----------------------------------------------------------------------------------------------------------
+struct Timer {
+    double ms = 0.0;
+    long count = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct galx_ctx {
+    galx_config cfg;
+    int N = 0, n = 0, n2 = 0, n3 = 0;
+    int nElems = 0, nSides = 0;
+    size_t tsz = 8;  // trace element size
+    hipStream_t stream = nullptr;
+    KernelSet ks;
+    Basis basis;
+    double lambda = 0.0;
+    // device buffers
+    double *dU = nullptr, *dK = nullptr, *dUt = nullptr, *dScratch = nullptr;
+    double *dGeo = nullptr, *dFgeo = nullptr;
+    int *dConn = nullptr, *dInv = nullptr;
+    void *dUtr[2] = {nullptr, nullptr}, *dGtr = nullptr, *dMats = nullptr;
+    double *dDtc = nullptr, *dDtv = nullptr, *dDtOut = nullptr;
+    int* dErr = nullptr;
+    int cur = 0;
+    bool meshSet = false, stateSet = false, utValid = false;
+    long long bytes = 0;
+    bool timing = false;
+    std::map<std::string, Timer> timers;
+};
+
+template <typename P>
+static int dalloc(galx_ctx* c, P** p, size_t bytes)
+{
+    if (*p) return GALX_OK;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(GALX_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    *p = (P*)q;
+    c->bytes += (long long)bytes;
+    return GALX_OK;
+}
+
+static void dfree(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+struct TimerScope {
+    galx_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimerScope(galx_ctx* c_, const char* n) : c(c_), name(n)
+    {
+        if (c->timing) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~TimerScope()
+    {
+        if (c->timing) {
+            (void)hipEventRecord(b, c->stream);
+            c->timers[name].pending.emplace_back(a, b);
+        }
+    }
+};
+
+static void flush_timers(galx_ctx* c)
+{
+    for (auto& kv : c->timers) {
+        for (auto& pr : kv.second.pending) {
+            float ms = 0.f;
+            (void)hipEventSynchronize(pr.second);
+            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) kv.second.ms += ms;
+            kv.second.count++;
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+        kv.second.pending.clear();
+    }
+}
+
+static Mesh mesh_of(galx_ctx* c) { return Mesh{c->dGeo, c->dFgeo, c->dConn, c->dInv}; }
+static GasD gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda}; }
+
+extern "C" {
+
+const char* galx_version(void) { return "galx 0.1 (gfx950)"; }
+
+unsigned galx_supported_degrees(int precision)
+{
+    unsigned m = 0;
+    if (precision == GALX_PREC_FP64) {
+#define X(NN) m |= 1u << NN;
+        GALX_DEGREES_FP64
+#undef X
+    } else {
+#define X(NN) m |= 1u << NN;
+        GALX_DEGREES_MIXED
+#undef X
+    }
+    return m;
+}
+
+const char* galx_last_error(void) { return g_err.c_str(); }
+
+int galx_create(const galx_config* cfg, galx_ctx** out)
+{
+    if (!cfg || !out) return fail(GALX_E_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->form != GALX_FORM_WEAK && cfg->form != GALX_FORM_SPLIT_PI) return fail(GALX_E_ARG, "bad form %d", cfg->form);
+    if (cfg->riemann != GALX_RIEMANN_LF && cfg->riemann != GALX_RIEMANN_ROE && cfg->riemann != GALX_RIEMANN_ROEEF)
+        return fail(GALX_E_ARG, "bad riemann %d", cfg->riemann);
+    if (cfg->precision != GALX_PREC_FP64 && cfg->precision != GALX_PREC_MIXED)
+        return fail(GALX_E_ARG, "bad precision %d", cfg->precision);
+    if (!(cfg->kappa > 1.0) || !(cfg->R > 0.0) || !(cfg->Pr > 0.0) || !(cfg->mu0 >= 0.0))
+        return fail(GALX_E_ARG, "bad gas constants");
+    KernelSet ks = select_kernels(cfg->N, cfg->precision, cfg->form);
+    if (!ks.ok) return fail(GALX_E_ARG, "degree N=%d not compiled for precision %d", cfg->N, cfg->precision);
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(GALX_E_ARG, "device %d not present (%d)", cfg->device, ndev);
+    HIPCHK(hipSetDevice(cfg->device));
+    galx_ctx* c = new galx_ctx();
+    c->cfg = *cfg;
+    c->N = cfg->N;
+    c->n = cfg->N + 1;
+    c->n2 = c->n * c->n;
+    c->n3 = c->n2 * c->n;
+    c->ks = ks;
+    c->tsz = cfg->precision == GALX_PREC_FP64 ? 8 : 4;
+    c->basis = make_basis(cfg->N);
+    // THERMAL_CONDUCTIVITY_EOS (src/equations/navierstokes/idealgas/eos.h:110)
+    c->lambda = cfg->mu0 * cfg->R * cfg->kappa / ((cfg->kappa - 1.0) * cfg->Pr);
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(GALX_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    // matrices for the kernels in the compute precision
+    int nn = c->n2;
+    std::vector<double> md(3 * nn);
+    for (int x = 0; x < nn; x++) {
+        md[x] = c->basis.DT[x];
+        md[nn + x] = c->basis.DhT[x];
+        md[2 * nn + x] = c->basis.DVS[x];
+    }
+    int rc;
+    if (c->tsz == 8) {
+        if ((rc = dalloc(c, (double**)&c->dMats, 3 * nn * sizeof(double)))) { galx_destroy(c); return rc; }
+        e = hipMemcpy(c->dMats, md.data(), 3 * nn * sizeof(double), hipMemcpyHostToDevice);
+    } else {
+        std::vector<float> mf(md.begin(), md.end());
+        if ((rc = dalloc(c, (float**)&c->dMats, 3 * nn * sizeof(float)))) { galx_destroy(c); return rc; }
+        e = hipMemcpy(c->dMats, mf.data(), 3 * nn * sizeof(float), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        galx_destroy(c);
+        return fail(GALX_E_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return GALX_OK;
+}
+
+int galx_destroy(galx_ctx* c)
+{
+    if (!c) return GALX_OK;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    flush_timers(c);
+    void* ps[] = {c->dU, c->dK, c->dUt, c->dScratch, c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dUtr[0], c->dUtr[1],
+                  c->dGtr, c->dMats, c->dDtc, c->dDtv, c->dDtOut, c->dErr};
+    for (void* p : ps) dfree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GALX_OK;
+}
+
+int galx_set_mesh(galx_ctx* c, int nElems, int nSides, const int* E2S, const int* S2E, const int* S2V2,
+                  const double* Mf, const double* Mg, const double* Mh, const double* sJ, const double* nv,
+                  const double* t1, const double* t2, const double* se)
+{
+    if (!c) return fail(GALX_E_ARG, "null context");
+    if (c->meshSet) return fail(GALX_E_STATE, "mesh already set");
+    if (nElems < 1 || nSides < 1 || !E2S || !S2E || !S2V2 || !Mf || !Mg || !Mh || !sJ || !nv || !t1 || !t2 || !se)
+        return fail(GALX_E_ARG, "bad mesh arguments");
+    if ((long)nSides * 2 != (long)nElems * 6)
+        return fail(GALX_E_MESH, "periodic conforming mesh needs nSides = 3*nElems (got %d, %d)", nSides, nElems);
+    if ((long)nSides >= (1L << 28)) return fail(GALX_E_ARG, "too many sides");
+    const int n = c->n, n2 = c->n2, n3 = c->n3;
+    // S2V2 range and S2V2_inv (src/mesh/mappings.f90:154-166)
+    std::vector<int> inv(5 * 6 * n2, -1);
+    for (int ls = 1; ls <= 6; ls++)
+        for (int f = 0; f <= 4; f++)
+            for (int q = 0; q < n; q++)
+                for (int p = 0; p < n; p++) {
+                    int a = S2V2[0 + 2 * (p + n * (q + n * (f + 5 * (ls - 1))))];
+                    int b = S2V2[1 + 2 * (p + n * (q + n * (f + 5 * (ls - 1))))];
+                    if (a < 0 || a > c->N || b < 0 || b > c->N) return fail(GALX_E_MESH, "S2V2 out of range");
+                    inv[(f * 6 + ls - 1) * n2 + a + n * b] = p + n * q;
+                }
+    for (int x : inv)
+        if (x < 0) return fail(GALX_E_MESH, "S2V2 is not a bijection per (flip, locSide)");
+    // connectivity (src/flexi.h:106-115): every side has exactly one master (flip 0) and one slave
+    std::vector<int> conn(6L * nElems);
+    std::vector<int> seen(2L * nSides, 0);
+    for (int e = 0; e < nElems; e++)
+        for (int ls = 1; ls <= 6; ls++) {
+            const int* r = E2S + 3 * ((ls - 1) + 6L * e);
+            int side = r[0], flip = r[1], isMaster = r[2];
+            if (side < 1 || side > nSides || flip < 0 || flip > 4 || (isMaster != 0 && isMaster != 1) ||
+                (flip == 0) != (isMaster == 1))
+                return fail(GALX_E_MESH, "ElemToSide entry (elem %d, locSide %d) invalid", e + 1, ls);
+            const int* s = S2E + 5L * (side - 1);
+            if (flip == 0) {
+                if (s[0] != e + 1 || s[2] != ls) return fail(GALX_E_MESH, "master of side %d inconsistent", side);
+            } else {
+                if (s[1] != e + 1 || s[3] != ls || s[4] != flip)
+                    return fail(GALX_E_MESH, "slave of side %d inconsistent (nBCSides must be 0)", side);
+            }
+            if (seen[2L * (side - 1) + (flip ? 1 : 0)]++) return fail(GALX_E_MESH, "side %d visited twice", side);
+            conn[6L * e + ls - 1] = ((side - 1) << 3) | flip;
+        }
+    int rc;
+    long nDOF = (long)nElems * n3, nfn = (long)nSides * n2;
+    c->nElems = nElems;
+    c->nSides = nSides;
+    if ((rc = dalloc(c, &c->dU, 5 * nDOF * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dK, 5 * nDOF * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dGeo, 10 * nDOF * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dFgeo, 10 * nfn * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dConn, 6L * nElems * sizeof(int)))) return rc;
+    if ((rc = dalloc(c, &c->dInv, inv.size() * sizeof(int)))) return rc;
+    if ((rc = dalloc(c, (char**)&c->dUtr[0], 2 * 5 * nfn * c->tsz))) return rc;
+    if ((rc = dalloc(c, (char**)&c->dUtr[1], 2 * 5 * nfn * c->tsz))) return rc;
+    if ((rc = dalloc(c, (char**)&c->dGtr, 2 * 12 * nfn * c->tsz))) return rc;
+    if ((rc = dalloc(c, &c->dDtc, nElems * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dDtv, nElems * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dDtOut, 4 * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dErr, sizeof(int)))) return rc;
+    HIPCHK(hipMemcpy(c->dConn, conn.data(), conn.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->dInv, inv.data(), inv.size() * sizeof(int), hipMemcpyHostToDevice));
+    // geometry into element / side planes, in chunks through pinned staging
+    const long chunkE = 4096;
+    std::vector<double> buf;
+    buf.resize(10 * chunkE * n3);
+    for (long e0 = 0; e0 < nElems; e0 += chunkE) {
+        long ne = std::min(chunkE, (long)nElems - e0);
+        for (long e = 0; e < ne; e++) {
+            double* o = buf.data() + e * 10 * n3;
+            long ge = e0 + e;
+            for (int x = 0; x < n3; x++) {
+                long id = ge * n3 + x;
+                for (int d = 0; d < 3; d++) {
+                    o[d * n3 + x] = Mf[3 * id + d];
+                    o[(3 + d) * n3 + x] = Mg[3 * id + d];
+                    o[(6 + d) * n3 + x] = Mh[3 * id + d];
+                }
+                o[9 * n3 + x] = sJ[id];
+            }
+        }
+        HIPCHK(hipMemcpy(c->dGeo + e0 * 10 * n3, buf.data(), ne * 10 * n3 * sizeof(double), hipMemcpyHostToDevice));
+    }
+    const long chunkS = 16384;
+    buf.resize(10 * chunkS * n2);
+    for (long s0 = 0; s0 < nSides; s0 += chunkS) {
+        long ns = std::min(chunkS, (long)nSides - s0);
+        for (long s = 0; s < ns; s++) {
+            double* o = buf.data() + s * 10 * n2;
+            long gs = s0 + s;
+            for (int x = 0; x < n2; x++) {
+                long id = gs * n2 + x;
+                for (int d = 0; d < 3; d++) {
+                    o[d * n2 + x] = nv[3 * id + d];
+                    o[(3 + d) * n2 + x] = t1[3 * id + d];
+                    o[(6 + d) * n2 + x] = t2[3 * id + d];
+                }
+                o[9 * n2 + x] = se[id];
+            }
+        }
+        HIPCHK(hipMemcpy(c->dFgeo + s0 * 10 * n2, buf.data(), ns * 10 * n2 * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c->meshSet = true;
+    return GALX_OK;
+}
+
+static int after_state(galx_ctx* c)
+{
+    HIPCHK(hipMemsetAsync(c->dK, 0, 5L * c->nElems * c->n3 * sizeof(double), c->stream));
+    c->cur = 0;
+    {
+        TimerScope ts(c, "trace");
+        c->ks.trace(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[0]);
+    }
+    HIPCHK(hipGetLastError());
+    c->stateSet = true;
+    c->utValid = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+int galx_set_state_device(galx_ctx* c, const double* dU)
+{
+    if (!c || !dU) return fail(GALX_E_ARG, "null argument");
+    if (!c->meshSet) return fail(GALX_E_STATE, "set_mesh first");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    long nDOF = (long)c->nElems * c->n3;
+    k_ref_to_planes<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, dU, c->dU);
+    HIPCHK(hipGetLastError());
+    return after_state(c);
+}
+
+int galx_set_state(galx_ctx* c, const double* U)
+{
+    if (!c || !U) return fail(GALX_E_ARG, "null argument");
+    if (!c->meshSet) return fail(GALX_E_STATE, "set_mesh first");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    long nDOF = (long)c->nElems * c->n3;
+    int rc;
+    if ((rc = dalloc(c, &c->dScratch, 5 * nDOF * sizeof(double)))) return rc;
+    HIPCHK(hipMemcpyAsync(c->dScratch, U, 5 * nDOF * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    k_ref_to_planes<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dScratch, c->dU);
+    HIPCHK(hipGetLastError());
+    return after_state(c);
+}
+
+int galx_get_state_device(galx_ctx* c, double* dU)
+{
+    if (!c || !dU) return fail(GALX_E_ARG, "null argument");
+    if (!c->stateSet) return fail(GALX_E_STATE, "no state");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    long nDOF = (long)c->nElems * c->n3;
+    k_planes_to_ref<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dU, dU);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+int galx_get_state(galx_ctx* c, double* U)
+{
+    if (!c || !U) return fail(GALX_E_ARG, "null argument");
+    if (!c->stateSet) return fail(GALX_E_STATE, "no state");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    long nDOF = (long)c->nElems * c->n3;
+    int rc;
+    if ((rc = dalloc(c, &c->dScratch, 5 * nDOF * sizeof(double)))) return rc;
+    k_planes_to_ref<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dU, c->dScratch);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(U, c->dScratch, 5 * nDOF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+static void launch_lift(galx_ctx* c)
+{
+    TimerScope ts(c, "lift");
+    const Basis& B = c->basis;
+    c->ks.lift(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[c->cur], c->dGtr, c->dMats, gas_of(c), 1.0 / B.w[0],
+               1.0 / B.w[c->N]);
+}
+
+static void launch_rhs(galx_ctx* c, Stage st)
+{
+    TimerScope ts(c, "rhs");
+    const Basis& B = c->basis;
+    c->ks.rhs(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[c->cur], c->dGtr, c->dUtr[c->cur ^ 1], c->dUt, c->dK,
+              c->dMats, gas_of(c), c->cfg.riemann, 1.0 / B.w[0], 1.0 / B.w[c->N], st);
+}
+
+int galx_rhs(galx_ctx* c, double t)
+{
+    (void)t;
+    if (!c) return fail(GALX_E_ARG, "null context");
+    if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    int rc;
+    if ((rc = dalloc(c, &c->dUt, 5L * c->nElems * c->n3 * sizeof(double)))) return rc;
+    launch_lift(c);
+    HIPCHK(hipGetLastError());
+    launch_rhs(c, Stage{0.0, 0.0, 0});
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->utValid = true;
+    return GALX_OK;
+}
+
+int galx_get_ut_device(galx_ctx* c, double* dUt)
+{
+    if (!c || !dUt) return fail(GALX_E_ARG, "null argument");
+    if (!c->utValid) return fail(GALX_E_STATE, "call galx_rhs first");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    long nDOF = (long)c->nElems * c->n3;
+    k_planes_to_ref<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUt, dUt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+int galx_get_ut(galx_ctx* c, double* Ut)
+{
+    if (!c || !Ut) return fail(GALX_E_ARG, "null argument");
+    if (!c->utValid) return fail(GALX_E_STATE, "call galx_rhs first");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    long nDOF = (long)c->nElems * c->n3;
+    int rc;
+    if ((rc = dalloc(c, &c->dScratch, 5 * nDOF * sizeof(double)))) return rc;
+    k_planes_to_ref<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUt, c->dScratch);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(Ut, c->dScratch, 5 * nDOF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+// Carpenter-Kennedy RK4-5 (src/timedisc/timedisc_vars.f90:213-227)
+static const double RKA[5] = {0.0, 567301805773.0 / 1357537059087.0, 2404267990393.0 / 2016746695238.0,
+                              3550918686646.0 / 2091501179385.0, 1275806237668.0 / 842570457699.0};
+static const double RKB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
+                              1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
+                              2277821191437.0 / 14882151754819.0};
+
+static int lserk_enqueue(galx_ctx* c, double dt, int nSteps)
+{
+    for (int s = 0; s < nSteps; s++)
+        for (int stage = 0; stage < 5; stage++) {
+            launch_lift(c);
+            double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
+            launch_rhs(c, Stage{mRKA, RKB[stage] * dt, 1});
+            c->cur ^= 1;
+        }
+    HIPCHK(hipGetLastError());
+    c->utValid = false;
+    return GALX_OK;
+}
+
+int galx_lserk_steps(galx_ctx* c, double t, double dt, int nSteps)
+{
+    (void)t;
+    if (!c) return fail(GALX_E_ARG, "null context");
+    if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
+    if (nSteps < 0 || !(dt == dt)) return fail(GALX_E_ARG, "bad step arguments");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    int rc = lserk_enqueue(c, dt, nSteps);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+int galx_lserk_step(galx_ctx* c, double t, double dt) { return galx_lserk_steps(c, t, dt, 1); }
+
+// CFLScaleAlpha / DFLScaleAlpha for Gauss-Lobatto nodes, Carpenter RK4-5 (src/timedisc/timedisc_vars.f90:105-109,
+// 188-227); scaling as FillCFL_DFL (src/timedisc/timedisc_func.f90:480-550)
+static const double CFL_ALPHA_GL[15] = {4.7497, 3.4144, 2.8451, 2.4739, 2.2027, 1.9912, 1.8225, 1.6830,
+                                        1.5682, 1.4692, 1.3849, 1.3106, 1.2454, 1.1880, 1.1362};
+static const double DFL_ALPHA_GL[10] = {4.50, 1.95, 1.18, 0.79, 0.56, 0.42, 0.32, 0.25, 0.20, 0.17};
+
+int galx_calc_dt(galx_ctx* c, double* dt, int* errType)
+{
+    if (!c || !dt || !errType) return fail(GALX_E_ARG, "null argument");
+    if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    int N = c->N;
+    int Nc = N < 15 ? N : 15, Nd = N < 10 ? N : 10;
+    double CFL = c->cfg.CFLScale * CFL_ALPHA_GL[Nc - 1] / (2.0 * N + 1.0);
+    double DFL = c->cfg.DFLScale * DFL_ALPHA_GL[Nd - 1] * 1.853 / ((2.0 * N + 1.0) * (2.0 * N + 1.0));
+    double kp = c->cfg.kappa / c->cfg.Pr;
+    double KPmax = kp > 4.0 / 3.0 ? kp : 4.0 / 3.0;
+    HIPCHK(hipMemsetAsync(c->dErr, 0, sizeof(int), c->stream));
+    {
+        TimerScope ts(c, "dt");
+        c->ks.dt(c->stream, c->nElems, c->dU, c->dGeo, c->cfg.kappa, c->cfg.mu0, KPmax, CFL, DFL, c->dDtc, c->dDtv,
+                 c->dErr);
+        k_dt_reduce<<<dim3(1), dim3(1024), 0, c->stream>>>(c->nElems, c->dDtc, c->dDtv, c->dErr, c->dDtOut);
+    }
+    HIPCHK(hipGetLastError());
+    double out[4];
+    HIPCHK(hipMemcpyAsync(out, c->dDtOut, sizeof(out), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *dt = out[2];
+    *errType = (int)out[3];
+    return GALX_OK;
+}
+
+int galx_timers_enable(galx_ctx* c, int on)
+{
+    if (!c) return fail(GALX_E_ARG, "null context");
+    c->timing = on != 0;
+    return GALX_OK;
+}
+
+int galx_timer_read(galx_ctx* c, const char* name, double* ms, long* count)
+{
+    if (!c || !name || !ms || !count) return fail(GALX_E_ARG, "null argument");
+    flush_timers(c);
+    auto it = c->timers.find(name);
+    *ms = it == c->timers.end() ? 0.0 : it->second.ms;
+    *count = it == c->timers.end() ? 0 : it->second.count;
+    return GALX_OK;
+}
+
+int galx_timers_reset(galx_ctx* c)
+{
+    if (!c) return fail(GALX_E_ARG, "null context");
+    flush_timers(c);
+    c->timers.clear();
+    return GALX_OK;
+}
+
+int galx_memory_bytes(galx_ctx* c, long long* bytes)
+{
+    if (!c || !bytes) return fail(GALX_E_ARG, "null argument");
+    *bytes = c->bytes;
+    return GALX_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------------------------------------------------
+// reference-layout unit kernels
+// ----------------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void face_to_vol_rt(int ls, int a, int b, int N, int& i, int& j, int& k)
+{
+    switch (ls) {
+    case 5: i = 0; j = a; k = b; break;
+    case 3: i = N; j = a; k = b; break;
+    case 2: i = a; j = 0; k = b; break;
+    case 4: i = a; j = N; k = b; break;
+    case 1: i = a; j = b; k = 0; break;
+    default: i = a; j = b; k = N; break;
+    }
+}
+
+// EvalElemFaceGL_Device (src/interpolation/prolongtoface.t90:268-344)
+__global__ void k_unit_prolong(int nVar, int N, int nElems, const int* __restrict__ E2S, const int* __restrict__ S2V2,
+                               const double* __restrict__ Uvol, double* __restrict__ Um, double* __restrict__ Us)
+{
+    int n = N + 1, n2 = n * n;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nElems * 6 * n2) return;
+    int e = (int)(t / (6 * n2));
+    int rest = (int)(t - (long)e * 6 * n2);
+    int ls = rest / n2 + 1;
+    rest -= (ls - 1) * n2;
+    int p = rest / n, q = rest % n;
+    int side = E2S[0 + 3 * (ls - 1 + 6 * e)] - 1, flip = E2S[1 + 3 * (ls - 1 + 6 * e)];
+    int a = S2V2[0 + 2 * (p + n * (q + n * (flip + 5 * (ls - 1))))];
+    int b = S2V2[1 + 2 * (p + n * (q + n * (flip + 5 * (ls - 1))))];
+    int i, j, k;
+    face_to_vol_rt(ls, a, b, N, i, j, k);
+    double* dst = flip == 0 ? Um : Us;
+    for (int v = 0; v < nVar; v++)
+        dst[v + nVar * (p + n * (q + (long)n * side))] = Uvol[v + nVar * (i + n * (j + n * (k + (long)n * e)))];
+}
+
+// SurfInt_Device, GL non-EXACT_MM (src/dg/surfint.t90:352-586)
+__global__ void k_unit_surfint(int nVar, int N, int nElems, const int* __restrict__ E2S, const int* __restrict__ inv,
+                               const double* __restrict__ Fm, const double* __restrict__ Fs, double LHm, double LHp,
+                               double* __restrict__ Ut)
+{
+    int n = N + 1;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nElems * n * n * n) return;
+    int e = (int)(t / (n * n * n));
+    int r = (int)(t - (long)e * n * n * n);
+    int i = r % n, j = (r / n) % n, k = r / (n * n);
+    double loc[16];
+    for (int v = 0; v < nVar; v++) loc[v] = 0.0;
+    for (int ls = 1; ls <= 6; ls++) {
+        int side = E2S[0 + 3 * (ls - 1 + 6 * e)] - 1, flip = E2S[1 + 3 * (ls - 1 + 6 * e)];
+        int a, b;
+        bool on;
+        double LH;
+        switch (ls) {
+        case 5: on = i == 0; a = j; b = k; LH = LHm; break;
+        case 2: on = j == 0; a = i; b = k; LH = LHm; break;
+        case 1: on = k == 0; a = i; b = j; LH = LHm; break;
+        case 3: on = i == N; a = j; b = k; LH = LHp; break;
+        case 4: on = j == N; a = i; b = k; LH = LHp; break;
+        default: on = k == N; a = i; b = j; LH = LHp; break;
+        }
+        if (!on) continue;
+        int pq = inv[(flip * 6 + ls - 1) * n * n + a + n * b];
+        long fo = (long)nVar * (pq + (long)n * n * side);
+        for (int v = 0; v < nVar; v++) {
+            if (flip == 0) loc[v] = loc[v] + Fm[fo + v] * LH;
+            else loc[v] = loc[v] - Fs[fo + v] * LH;
+        }
+    }
+    for (int v = 0; v < nVar; v++) {
+        double* u = &Ut[v + nVar * (long)t];
+        *u = *u + loc[v];
+    }
+}
+
+namespace {
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+int build_e2s_from_s2e(int nElems, int nSides, const int* S2E, std::vector<int>& E2S)
+{
+    E2S.assign(18L * nElems, 0);
+    std::vector<int> filled(6L * nElems, 0);
+    for (int s = 0; s < nSides; s++) {
+        const int* r = S2E + 5L * s;
+        if (r[0] > 0) {
+            if (r[0] > nElems || r[2] < 1 || r[2] > 6) return fail(GALX_E_MESH, "SideToElem master of side %d", s + 1);
+            int* o = &E2S[3 * (r[2] - 1 + 6L * (r[0] - 1))];
+            o[0] = s + 1; o[1] = 0; o[2] = 1;
+            filled[6L * (r[0] - 1) + r[2] - 1]++;
+        }
+        if (r[1] > 0) {
+            if (r[1] > nElems || r[3] < 1 || r[3] > 6 || r[4] < 1 || r[4] > 4)
+                return fail(GALX_E_MESH, "SideToElem slave of side %d", s + 1);
+            int* o = &E2S[3 * (r[3] - 1 + 6L * (r[1] - 1))];
+            o[0] = s + 1; o[1] = r[4]; o[2] = 0;
+            filled[6L * (r[1] - 1) + r[3] - 1]++;
+        }
+    }
+    for (int f : filled)
+        if (f != 1) return fail(GALX_E_MESH, "every element side must appear exactly once in SideToElem");
+    return GALX_OK;
+}
+}  // namespace
+
+extern "C" int galx_prolong_to_face(int device, int nVar, int N, int nElems, int nSides, const int* S2E,
+                                    const int* S2V2, const double* Uvol, double* Um, double* Us)
+{
+    if (nVar < 1 || nVar > 16 || N < 1 || N > 30 || nElems < 1 || nSides < 1 || !S2E || !S2V2 || !Uvol || !Um || !Us)
+        return fail(GALX_E_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(device));
+    std::vector<int> E2S;
+    int rc = build_e2s_from_s2e(nElems, nSides, S2E, E2S);
+    if (rc) return rc;
+    int n = N + 1;
+    for (long x = 0; x < 2L * n * n * 30; x++)
+        if (S2V2[x] < 0 || S2V2[x] > N) return fail(GALX_E_MESH, "S2V2 out of range");
+    size_t sv = (size_t)nVar * n * n * n * nElems * 8, sf = (size_t)nVar * n * n * nSides * 8;
+    DevBuf dE, dS, dU, dM, dSl;
+    HIPCHK(hipMalloc(&dE.p, E2S.size() * 4));
+    HIPCHK(hipMalloc(&dS.p, 2L * n * n * 30 * 4));
+    HIPCHK(hipMalloc(&dU.p, sv));
+    HIPCHK(hipMalloc(&dM.p, sf));
+    HIPCHK(hipMalloc(&dSl.p, sf));
+    HIPCHK(hipMemcpy(dE.p, E2S.data(), E2S.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dS.p, S2V2, 2L * n * n * 30 * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dU.p, Uvol, sv, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dM.p, Um, sf, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dSl.p, Us, sf, hipMemcpyHostToDevice));
+    long nt = (long)nElems * 6 * n * n;
+    k_unit_prolong<<<dim3((unsigned)((nt + 127) / 128)), dim3(128)>>>(nVar, N, nElems, (int*)dE.p, (int*)dS.p,
+                                                                     (double*)dU.p, (double*)dM.p, (double*)dSl.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(Um, dM.p, sf, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(Us, dSl.p, sf, hipMemcpyDeviceToHost));
+    return GALX_OK;
+}
+
+extern "C" int galx_surf_int(int device, int nVar, int N, int nElems, int nSides, const int* S2E, const int* S2V2,
+                             const double* Fm, const double* Fs, double LHm, double LHp, double* Ut)
+{
+    if (nVar < 1 || nVar > 16 || N < 1 || N > 30 || nElems < 1 || nSides < 1 || !S2E || !S2V2 || !Fm || !Fs || !Ut)
+        return fail(GALX_E_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(device));
+    std::vector<int> E2S;
+    int rc = build_e2s_from_s2e(nElems, nSides, S2E, E2S);
+    if (rc) return rc;
+    int n = N + 1, n2 = n * n;
+    std::vector<int> inv(5 * 6 * n2, -1);
+    for (int ls = 1; ls <= 6; ls++)
+        for (int f = 0; f <= 4; f++)
+            for (int q = 0; q < n; q++)
+                for (int p = 0; p < n; p++) {
+                    int a = S2V2[0 + 2 * (p + n * (q + n * (f + 5 * (ls - 1))))];
+                    int b = S2V2[1 + 2 * (p + n * (q + n * (f + 5 * (ls - 1))))];
+                    if (a < 0 || a > N || b < 0 || b > N) return fail(GALX_E_MESH, "S2V2 out of range");
+                    inv[(f * 6 + ls - 1) * n2 + a + n * b] = p + n * q;
+                }
+    for (int x : inv)
+        if (x < 0) return fail(GALX_E_MESH, "S2V2 is not a bijection");
+    size_t sv = (size_t)nVar * n * n * n * nElems * 8, sf = (size_t)nVar * n2 * nSides * 8;
+    DevBuf dE, dI, dM, dSl, dU;
+    HIPCHK(hipMalloc(&dE.p, E2S.size() * 4));
+    HIPCHK(hipMalloc(&dI.p, inv.size() * 4));
+    HIPCHK(hipMalloc(&dM.p, sf));
+    HIPCHK(hipMalloc(&dSl.p, sf));
+    HIPCHK(hipMalloc(&dU.p, sv));
+    HIPCHK(hipMemcpy(dE.p, E2S.data(), E2S.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dI.p, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dM.p, Fm, sf, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dSl.p, Fs, sf, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dU.p, Ut, sv, hipMemcpyHostToDevice));
+    long nt = (long)nElems * n * n * n;
+    k_unit_surfint<<<dim3((unsigned)((nt + 127) / 128)), dim3(128)>>>(nVar, N, nElems, (int*)dE.p, (int*)dI.p,
+                                                                     (double*)dM.p, (double*)dSl.p, LHm, LHp,
+                                                                     (double*)dU.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(Ut, dU.p, sv, hipMemcpyDeviceToHost));
+    return GALX_OK;
+}

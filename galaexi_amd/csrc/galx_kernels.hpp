@@ -1,0 +1,621 @@
+// galx_kernels.hpp -- element-per-workgroup DGSEM kernels for gfx950.
+//
+// Data layout in HBM (one rank's element share on one GPU):
+//   U, Ut, Ut_tmp : double [nElems][5][n^3]          (element-major, variable planes, i fastest)
+//   geo           : double [nElems][10][n^3]         (Ja^1(3), Ja^2(3), Ja^3(3), sJ)
+//   fgeo          : double [nSides][10][n^2]         (NormVec(3), TangVec1(3), TangVec2(3), SurfElem), master (p,q)
+//   utr[2]        : TT [nSides][2][5][n^2]           U traces, slot 0 master / 1 slave, master (p,q); double buffered
+//   gtr           : TT [nSides][2][12][n^2]          lifted gradient traces (u,v,w,T) x (x,y,z)
+//   conn          : int [nElems][6]                  (SideID-1) << 3 | flip
+//   inv           : int [5 flips][6 locSides][n^2]   S2V2_inv as p + n*q
+// TT (trace type) is double in the FP64 form and float in the mixed form.
+//
+// Per RK stage: k_lift (BR1 gradients -> gradient traces) then k_rhs (gradients recomputed, viscous weak-form
+// volume term, split-form flux differencing or weak Euler term, Riemann + viscous face fluxes, surface integral,
+// -sJ scaling and the LSERK register update; writes U, Ut_tmp and the next U traces).
+#pragma once
+#include "galx_physics.hpp"
+
+namespace galx {
+
+template <int N>
+struct Dm {
+    static constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, nf = 6 * n2;
+    static constexpr int block = n3 <= 64 ? 64 : ((n3 + 63) / 64) * 64;
+    static constexpr int ft = (nf + block - 1) / block;
+};
+
+// volume node of face node (a,b) of local side ls (GL trace); ls numbering of src/flexi.h:98-103
+template <int N>
+__device__ __forceinline__ int face_vol_node(int ls, int a, int b)
+{
+    constexpr int n = N + 1;
+    switch (ls) {
+    case 5: return n * (a + n * b);
+    case 3: return N + n * (a + n * b);
+    case 2: return a + n * n * b;
+    case 4: return a + n * (N + n * b);
+    case 1: return a + n * b;
+    default: return a + n * (b + n * N);
+    }
+}
+
+// is node (i,j,k) on local side ls, and its face coordinates (a,b) there
+template <int N>
+__device__ __forceinline__ bool on_face(int ls, int i, int j, int k, int& ab)
+{
+    constexpr int n = N + 1;
+    switch (ls) {
+    case 1: ab = i + n * j; return k == 0;
+    case 2: ab = i + n * k; return j == 0;
+    case 3: ab = j + n * k; return i == N;
+    case 4: ab = i + n * k; return j == N;
+    case 5: ab = j + n * k; return i == 0;
+    default: ab = i + n * j; return k == N;
+    }
+}
+
+struct Mesh {
+    const double* geo;
+    const double* fgeo;
+    const int* conn;
+    const int* inv;
+};
+
+struct Stage {
+    double mRKA, bdt;
+    int mode;  // 0: write Ut; 1: LSERK register update
+};
+
+// matrices staged in LDS: D_T (transpose of D), D_Hat_T, DVolSurf, all (0:N,0:N) column-major
+template <int N, typename T>
+struct Mats {
+    T DT[(N + 1) * (N + 1)];
+    T DhT[(N + 1) * (N + 1)];
+    T DVS[(N + 1) * (N + 1)];
+};
+
+template <int N, typename T>
+__device__ __forceinline__ void load_mats(Mats<N, T>& M, const T* __restrict__ g, int tid, int nthreads)
+{
+    constexpr int nn = (N + 1) * (N + 1);
+    for (int x = tid; x < 3 * nn; x += nthreads) {
+        T v = g[x];
+        if (x < nn) M.DT[x] = v;
+        else if (x < 2 * nn) M.DhT[x - nn] = v;
+        else M.DVS[x - 2 * nn] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// BR1 lifting (src/dg/lifting/lifting_br1.t90:47-167, strong form, non-conservative volume integral
+// lifting_volint.t90:262-328, Lifting_FillFlux lifting_fillflux.t90:39-93, SurfInt_Device_Single_Flux with
+// doApplyJacobian surfint.t90:591-725).  Lifted variables (u,v,w,T); the density gradient of the reference's
+// nVarLifting=5 never enters the fluxes and is not formed.
+//   s_lift  : T[4][n3]   own (u,v,w,T), filled by the caller before the call (synchronised)
+//   s_lflux : T[6][7][n2] scratch
+// returns g[12] = grad(u,v,w,T) in x, then y, then z for node tid (valid if tid < n3).
+// ---------------------------------------------------------------------------------------------------------
+template <int N, typename T, typename TT>
+__device__ __forceinline__ void lift_gradient(int e, int tid, const Mesh& mesh, const TT* __restrict__ utr,
+                                              const Mats<N, T>& M, const T* s_lift, T* s_lflux, const Gas<T>& gas,
+                                              T LHm, T LHp, T* g)
+{
+    using D = Dm<N>;
+    constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
+    // face tasks: BR1 lifting flux 0.5*SurfElem*(-L_master + L_slave) and the normal
+    for (int t = tid; t < D::nf; t += D::block) {
+        int ls = t / n2 + 1, ab = t - (ls - 1) * n2, a = ab % n, b = ab / n;
+        int node = face_vol_node<N>(ls, a, b);
+        int c = mesh.conn[e * 6 + ls - 1];
+        long side = c >> 3;
+        int flip = c & 7;
+        int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
+        const TT* nb = utr + ((side * 2 + (flip ? 0 : 1)) * 5) * n2 + pq;
+        T uc[5], pr[6];
+#pragma unroll
+        for (int v = 0; v < 5; v++) uc[v] = T(nb[v * n2]);
+        cons_to_prim(uc, pr, gas);
+        T Lnb[4] = {pr[1], pr[2], pr[3], pr[5]};
+        const double* fg = mesh.fgeo + side * 10 * n2 + pq;
+        T se = T(fg[9 * n2]);
+        T* o = s_lflux + (ls - 1) * 7 * n2 + ab;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            T own = s_lift[q * n3 + node];
+            T Lm = flip ? Lnb[q] : own, Lsl = flip ? own : Lnb[q];
+            o[q * n2] = T(0.5) * se * (T(-1) * Lm + Lsl);
+        }
+        o[4 * n2] = T(fg[0]);
+        o[5 * n2] = T(fg[n2]);
+        o[6 * n2] = T(fg[2 * n2]);
+    }
+    __syncthreads();
+    if (tid < n3) {
+        int i = tid % n, j = (tid / n) % n, k = tid / n2;
+        T gxi[4] = {0, 0, 0, 0}, get[4] = {0, 0, 0, 0}, gze[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int l = 0; l <= N; l++) {
+            T di = M.DT[l + n * i], dj = M.DT[l + n * j], dk = M.DT[l + n * k];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                gxi[q] = gxi[q] + di * s_lift[q * n3 + l + n * (j + n * k)];
+                get[q] = get[q] + dj * s_lift[q * n3 + i + n * (l + n * k)];
+                gze[q] = gze[q] + dk * s_lift[q * n3 + i + n * (j + n * l)];
+            }
+        }
+        const double* ge = mesh.geo + (long)e * 10 * n3 + tid;
+        T mf[3] = {T(ge[0]), T(ge[n3]), T(ge[2 * n3])};
+        T mg[3] = {T(ge[3 * n3]), T(ge[4 * n3]), T(ge[5 * n3])};
+        T mh[3] = {T(ge[6 * n3]), T(ge[7 * n3]), T(ge[8 * n3])};
+        T sJ = T(ge[9 * n3]);
+        T loc[12];
+#pragma unroll
+        for (int q = 0; q < 12; q++) loc[q] = T(0);
+#pragma unroll
+        for (int ls = 1; ls <= 6; ls++) {
+            int ab;
+            if (!on_face<N>(ls, i, j, k, ab)) continue;
+            const T* o = s_lflux + (ls - 1) * 7 * n2 + ab;
+            T LH = (ls == 1 || ls == 2 || ls == 5) ? LHm : LHp;
+            T n0 = o[4 * n2], n1 = o[5 * n2], n2v = o[6 * n2];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                T lf = o[q * n2];
+                loc[q] = loc[q] + T(1) * (lf * n0) * LH;
+                loc[4 + q] = loc[4 + q] + T(1) * (lf * n1) * LH;
+                loc[8 + q] = loc[8 + q] + T(1) * (lf * n2v) * LH;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            T vx = mf[0] * gxi[q] + mg[0] * get[q] + mh[0] * gze[q];
+            T vy = mf[1] * gxi[q] + mg[1] * get[q] + mh[1] * gze[q];
+            T vz = mf[2] * gxi[q] + mg[2] * get[q] + mh[2] * gze[q];
+            g[q] = sJ * (vx + loc[q]);
+            g[4 + q] = sJ * (vy + loc[4 + q]);
+            g[8 + q] = sJ * (vz + loc[8 + q]);
+        }
+    }
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void load_state(int e, int tid, const double* __restrict__ U, double* ud, T* uc, T* pr,
+                                           const Gas<T>& gas)
+{
+    constexpr int n3 = Dm<N>::n3;
+    const double* u = U + (long)e * 5 * n3 + tid;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        ud[v] = u[v * n3];
+        uc[v] = T(ud[v]);
+    }
+    cons_to_prim(uc, pr, gas);
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// k_trace: U traces of all faces (ProlongToFaceCons, EvalElemFaceGL_Device prolongtoface.t90:268-344, in the
+// element-wise form: each element writes its own slot of each of its 6 sides).
+// ---------------------------------------------------------------------------------------------------------
+template <int N, typename TT>
+__global__ void __launch_bounds__(Dm<N>::block) k_trace(const double* __restrict__ U, Mesh mesh, TT* __restrict__ utr)
+{
+    using D = Dm<N>;
+    constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
+    const int e = blockIdx.x, tid = threadIdx.x;
+    for (int t = tid; t < D::nf; t += D::block) {
+        int ls = t / n2 + 1, ab = t - (ls - 1) * n2, a = ab % n, b = ab / n;
+        int node = face_vol_node<N>(ls, a, b);
+        int c = mesh.conn[e * 6 + ls - 1];
+        long side = c >> 3;
+        int flip = c & 7;
+        int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
+        TT* o = utr + ((side * 2 + (flip ? 1 : 0)) * 5) * n2 + pq;
+        const double* u = U + (long)e * 5 * n3 + node;
+#pragma unroll
+        for (int v = 0; v < 5; v++) o[v * n2] = TT(u[v * n3]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// k_lift: lifted gradients and their face traces (ProlongToFaceLifting lifting_br1.t90:152-156).
+// ---------------------------------------------------------------------------------------------------------
+template <int N, typename T, typename TT>
+__global__ void __launch_bounds__(Dm<N>::block)
+    k_lift(const double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, TT* __restrict__ gtr,
+           const T* __restrict__ mats, Gas<T> gas, T LHm, T LHp)
+{
+    using D = Dm<N>;
+    constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
+    constexpr int lds_a = 4 * n3 + 6 * 7 * n2, lds_b = 6 * 12 * n2;
+    __shared__ T s_buf[lds_a > lds_b ? lds_a : lds_b];
+    __shared__ Mats<N, T> M;
+    const int e = blockIdx.x, tid = threadIdx.x;
+    T* s_lift = s_buf;
+    T* s_lflux = s_buf + 4 * n3;
+    load_mats<N, T>(M, mats, tid, D::block);
+    double ud[5];
+    T uc[5], pr[6];
+    if (tid < n3) {
+        load_state<N, T>(e, tid, U, ud, uc, pr, gas);
+        s_lift[tid] = pr[1];
+        s_lift[n3 + tid] = pr[2];
+        s_lift[2 * n3 + tid] = pr[3];
+        s_lift[3 * n3 + tid] = pr[5];
+    }
+    __syncthreads();
+    T g[12];
+    lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_lift, s_lflux, gas, LHm, LHp, g);
+    __syncthreads();
+    T* s_out = s_buf;  // [6][12][n2]
+    if (tid < n3) {
+        int i = tid % n, j = (tid / n) % n, k = tid / n2;
+#pragma unroll
+        for (int ls = 1; ls <= 6; ls++) {
+            int ab;
+            if (!on_face<N>(ls, i, j, k, ab)) continue;
+            T* o = s_out + (ls - 1) * 12 * n2 + ab;
+#pragma unroll
+            for (int q = 0; q < 12; q++) o[q * n2] = g[q];
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < D::nf; t += D::block) {
+        int ls = t / n2 + 1, ab = t - (ls - 1) * n2;
+        int c = mesh.conn[e * 6 + ls - 1];
+        long side = c >> 3;
+        int flip = c & 7;
+        int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
+        TT* o = gtr + ((side * 2 + (flip ? 1 : 0)) * 12) * n2 + pq;
+        const T* s = s_out + (ls - 1) * 12 * n2 + ab;
+#pragma unroll
+        for (int q = 0; q < 12; q++) o[q * n2] = TT(s[q * n2]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// k_rhs: DGTimeDerivative_weakForm (src/dg/dg.f90:255-425) after the lifting, fused with the LSERK register
+// update VAXPB_OUT_VAXPB_IN (src/globals/vector.f90:163-183, src/timedisc/timestep.f90:85-119).
+//   SPLIT: VolInt_weakForm_Visc + VolInt_splitForm_Kernel (src/dg/volint.f90:60-119,265-353)
+//   else : VolInt_weakForm with EvalTransformedEulerDiffFlux3D (src/dg/volint.f90:129-188)
+//   FillFlux: Riemann + ViscousFlux + SurfElem (src/dg/fillflux.f90:45-172), SurfInt_Device (surfint.t90:352-586)
+//   Ut = -sJ*Ut (src/dg/dg.f90:413,423)
+// ---------------------------------------------------------------------------------------------------------
+template <int N, typename T, typename TT, bool SPLIT>
+__global__ void __launch_bounds__(Dm<N>::block)
+    k_rhs(double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, const TT* __restrict__ gtr,
+          TT* __restrict__ utr_next, double* __restrict__ Ut, double* __restrict__ Kreg, const T* __restrict__ mats,
+          Gas<T> gas, int riemann, T LHm, T LHp, Stage st)
+{
+    using D = Dm<N>;
+    constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
+    constexpr int lds_a = 4 * n3 + 6 * 7 * n2;
+    constexpr int lds_c = SPLIT ? 9 * n3 : 5 * n3;
+    constexpr int lds_d = 6 * 17 * n2;
+    constexpr int lds_ac = lds_a > lds_c ? lds_a : lds_c;
+    __shared__ T s_buf[lds_ac > lds_d ? lds_ac : lds_d];
+    __shared__ Mats<N, T> M;
+    const int e = blockIdx.x, tid = threadIdx.x;
+    const bool active = tid < n3;
+    const int i = tid % n, j = (tid / n) % n, k = tid / n2;
+    load_mats<N, T>(M, mats, tid, D::block);
+    double ud[5];
+    T uc[5], pr[6], g[12];
+    T* s_lift = s_buf;
+    if (active) {
+        load_state<N, T>(e, tid, U, ud, uc, pr, gas);
+        s_lift[tid] = pr[1];
+        s_lift[n3 + tid] = pr[2];
+        s_lift[2 * n3 + tid] = pr[3];
+        s_lift[3 * n3 + tid] = pr[5];
+    }
+    __syncthreads();
+    lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_lift, s_buf + 4 * n3, gas, LHm, LHp, g);
+    __syncthreads();
+
+    // ---- viscous (split) or full (weak) transformed fluxes, weak-form D_Hat_T contraction ----
+    T ut[5] = {0, 0, 0, 0, 0};
+    T ja[9];
+    double sJd = 0.0;
+    Stress<T> s;
+    if (active) {
+        const double* ge = mesh.geo + (long)e * 10 * n3 + tid;
+#pragma unroll
+        for (int q = 0; q < 9; q++) ja[q] = T(ge[q * n3]);
+        sJd = ge[9 * n3];
+        stress(pr, g, gas, s);
+    }
+    T* s_f = s_buf;  // [5][n3]
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (active) {
+            T f[5];
+            transformed_flux<T, !SPLIT>(uc, pr, s, ja + 3 * d, f);
+#pragma unroll
+            for (int v = 0; v < 5; v++) s_f[v * n3 + tid] = f[v];
+        }
+        __syncthreads();
+        if (active) {
+            int idx = d == 0 ? i : (d == 1 ? j : k);
+            int stride = d == 0 ? 1 : (d == 1 ? n : n2);
+            int base = tid - idx * stride;
+#pragma unroll
+            for (int l = 0; l <= N; l++) {
+                T dh = M.DhT[l + n * idx];
+#pragma unroll
+                for (int v = (SPLIT ? 1 : 0); v < 5; v++) ut[v] = ut[v] + dh * s_f[v * n3 + base + l * stride];
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- split-form flux differencing (VolInt_splitForm_Kernel) ----
+    if (SPLIT) {
+        T* s_v = s_buf;           // [6][n3] rho,u,v,w,p,H
+        T* s_m = s_buf + 6 * n3;  // [3][n3] metric of the current direction
+        if (active) {
+            s_v[tid] = pr[0];
+            s_v[n3 + tid] = pr[1];
+            s_v[2 * n3 + tid] = pr[2];
+            s_v[3 * n3 + tid] = pr[3];
+            s_v[4 * n3 + tid] = pr[4];
+            s_v[5 * n3 + tid] = (uc[4] + pr[4]) / uc[0];
+        }
+        T tmp[5] = {0, 0, 0, 0, 0};
+        T Hme = active ? (uc[4] + pr[4]) / uc[0] : T(0);
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (active) {
+                s_m[tid] = ja[3 * d];
+                s_m[n3 + tid] = ja[3 * d + 1];
+                s_m[2 * n3 + tid] = ja[3 * d + 2];
+            }
+            __syncthreads();
+            if (active) {
+                int idx = d == 0 ? i : (d == 1 ? j : k);
+                int stride = d == 0 ? 1 : (d == 1 ? n : n2);
+                int base = tid - idx * stride;
+#pragma unroll
+                for (int l = 0; l <= N; l++) {
+                    if (l == idx) continue;  // DVolSurf diagonal vanishes for Gauss-Lobatto (src/dg/dg.f90:229-237)
+                    int o = base + l * stride;
+                    T Mo[3] = {s_m[o], s_m[n3 + o], s_m[2 * n3 + o]};
+                    T F[5];
+                    split_volume_flux_pi<T>(pr[0], pr[1], pr[2], pr[3], pr[4], Hme, s_v[o], s_v[n3 + o],
+                                            s_v[2 * n3 + o], s_v[3 * n3 + o], s_v[4 * n3 + o], s_v[5 * n3 + o],
+                                            ja + 3 * d, Mo, F);
+                    T dv = M.DVS[l + n * idx];
+#pragma unroll
+                    for (int v = 0; v < 5; v++) tmp[v] = tmp[v] + dv * F[v];
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++) ut[v] = ut[v] + tmp[v];
+    }
+
+    // ---- face fluxes (FillFlux) and surface integral (SurfInt_Device) ----
+    T* s_face = s_buf;  // [6][17][n2]: own cons(5), own grad(12); later results [6][5][n2]
+    if (active) {
+#pragma unroll
+        for (int ls = 1; ls <= 6; ls++) {
+            int ab;
+            if (!on_face<N>(ls, i, j, k, ab)) continue;
+            T* o = s_face + (ls - 1) * 17 * n2 + ab;
+#pragma unroll
+            for (int v = 0; v < 5; v++) o[v * n2] = uc[v];
+#pragma unroll
+            for (int q = 0; q < 12; q++) o[(5 + q) * n2] = g[q];
+        }
+    }
+    __syncthreads();
+    T res[D::ft][5];
+#pragma unroll
+    for (int r = 0; r < D::ft; r++) {
+        int t = tid + r * D::block;
+        if (t >= D::nf) continue;
+        int ls = t / n2 + 1, ab = t - (ls - 1) * n2;
+        int c = mesh.conn[e * 6 + ls - 1];
+        long side = c >> 3;
+        int flip = c & 7;
+        int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
+        const T* so = s_face + (ls - 1) * 17 * n2 + ab;
+        T Uo[5], Po[6], Go[12], Un[5], Pn[6], Gn[12];
+#pragma unroll
+        for (int v = 0; v < 5; v++) Uo[v] = so[v * n2];
+#pragma unroll
+        for (int q = 0; q < 12; q++) Go[q] = so[(5 + q) * n2];
+        const long nslot = side * 2 + (flip ? 0 : 1);
+        const TT* un = utr + nslot * 5 * n2 + pq;
+        const TT* gn = gtr + nslot * 12 * n2 + pq;
+#pragma unroll
+        for (int v = 0; v < 5; v++) Un[v] = T(un[v * n2]);
+#pragma unroll
+        for (int q = 0; q < 12; q++) Gn[q] = T(gn[q * n2]);
+        cons_to_prim(Uo, Po, gas);
+        cons_to_prim(Un, Pn, gas);
+        const double* fg = mesh.fgeo + side * 10 * n2 + pq;
+        T nv[3] = {T(fg[0]), T(fg[n2]), T(fg[2 * n2])};
+        T t1[3] = {T(fg[3 * n2]), T(fg[4 * n2]), T(fg[5 * n2])};
+        T t2[3] = {T(fg[6 * n2]), T(fg[7 * n2]), T(fg[8 * n2])};
+        T se = T(fg[9 * n2]);
+        const T *UL = flip ? Un : Uo, *UR = flip ? Uo : Un, *PL = flip ? Pn : Po, *PR = flip ? Po : Pn;
+        const T *GL = flip ? Gn : Go, *GR = flip ? Go : Gn;
+        T F[5], fv[5];
+        riemann_point<T, SPLIT>(riemann, UL, UR, PL, PR, nv, t1, t2, gas.kappa, F);
+        normal_diff_flux(PL, GL, nv, gas, fv);
+#pragma unroll
+        for (int v = 0; v < 5; v++) F[v] = F[v] + T(0.5) * fv[v];
+        normal_diff_flux(PR, GR, nv, gas, fv);
+#pragma unroll
+        for (int v = 0; v < 5; v++) F[v] = F[v] + T(0.5) * fv[v];
+        T sg = flip ? T(-1) : T(1);
+#pragma unroll
+        for (int v = 0; v < 5; v++) res[r][v] = sg * (F[v] * se);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < D::ft; r++) {
+        int t = tid + r * D::block;
+        if (t >= D::nf) continue;
+        int ls = t / n2 + 1, ab = t - (ls - 1) * n2;
+#pragma unroll
+        for (int v = 0; v < 5; v++) s_face[(ls - 1) * 5 * n2 + v * n2 + ab] = res[r][v];
+    }
+    __syncthreads();
+    double unew[5];
+    if (active) {
+        T loc[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int ls = 1; ls <= 6; ls++) {
+            int ab;
+            if (!on_face<N>(ls, i, j, k, ab)) continue;
+            T LH = (ls == 1 || ls == 2 || ls == 5) ? LHm : LHp;
+#pragma unroll
+            for (int v = 0; v < 5; v++) loc[v] = loc[v] + s_face[(ls - 1) * 5 * n2 + v * n2 + ab] * LH;
+        }
+        double* ut_out = (st.mode == 0 ? Ut : Kreg) + (long)e * 5 * n3 + tid;
+        double* uo = U + (long)e * 5 * n3 + tid;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            T tot = ut[v] + loc[v];
+            double utd = double(tot * T(-1)) * sJd;
+            if (st.mode == 0) {
+                ut_out[v * n3] = utd;
+            } else {
+                double kk = ut_out[v * n3] * st.mRKA + utd;
+                ut_out[v * n3] = kk;
+                unew[v] = ud[v] + kk * st.bdt;
+                uo[v * n3] = unew[v];
+            }
+        }
+    }
+    if (st.mode == 0) return;
+    // ---- next U traces ----
+    __syncthreads();
+    T* s_u = s_buf;  // [6][5][n2] new U on the faces, rounded to the trace type
+    if (active) {
+#pragma unroll
+        for (int ls = 1; ls <= 6; ls++) {
+            int ab;
+            if (!on_face<N>(ls, i, j, k, ab)) continue;
+#pragma unroll
+            for (int v = 0; v < 5; v++) s_u[(ls - 1) * 5 * n2 + v * n2 + ab] = T(TT(unew[v]));
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < D::nf; t += D::block) {
+        int ls = t / n2 + 1, ab = t - (ls - 1) * n2;
+        int c = mesh.conn[e * 6 + ls - 1];
+        long side = c >> 3;
+        int flip = c & 7;
+        int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
+        TT* o = utr_next + ((side * 2 + (flip ? 1 : 0)) * 5) * n2 + pq;
+#pragma unroll
+        for (int v = 0; v < 5; v++) o[v * n2] = TT(s_u[(ls - 1) * 5 * n2 + v * n2 + ab]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// time step: CalcMaxEigenvalue + CALCTIMESTEP (src/equations/navierstokes/calctimestep.f90:51-296), FP64
+// ---------------------------------------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(Dm<N>::block)
+    k_dt_elem(const double* __restrict__ U, const double* __restrict__ geo, double kappa, double mu0, double KPmax,
+              double CFLScale, double DFLScale, double* __restrict__ dtc, double* __restrict__ dtv,
+              int* __restrict__ err)
+{
+    using D = Dm<N>;
+    constexpr int n3 = D::n3;
+    constexpr int nw = D::block / 64;
+    __shared__ double s_red[6][nw];
+    const int e = blockIdx.x, tid = threadIdx.x;
+    double lam[6] = {0, 0, 0, 0, 0, 0};
+    int bad = 0;
+    if (tid < n3) {
+        const double* u = U + (long)e * 5 * n3 + tid;
+        const double* ge = geo + (long)e * 10 * n3 + tid;
+        double rho = u[0], srho = 1.0 / rho;
+        double vel[3] = {u[n3] * srho, u[2 * n3] * srho, u[3 * n3] * srho};
+        double p = (kappa - 1.0) * (u[4 * n3] - 0.5 * (vel[0] * u[n3] + vel[1] * u[2 * n3] + vel[2] * u[3 * n3]));
+        double c = sqrt(kappa * p * srho);
+        double sj = ge[9 * n3];
+        double vsJ[3] = {vel[0] * sj, vel[1] * sj, vel[2] * sj};
+        if (!(rho > 0.0)) bad = 1;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double m0 = ge[(3 * d) * n3], m1 = ge[(3 * d + 1) * n3], m2 = ge[(3 * d + 2) * n3];
+            double adv = sj * sqrt(m0 * m0 + m1 * m1 + m2 * m2);
+            double lc = fabs(m0 * vsJ[0] + m1 * vsJ[1] + m2 * vsJ[2]) + c * adv;
+            double a0 = m0 * sj, a1 = m1 * sj, a2 = m2 * sj;
+            double lv = mu0 * srho * (KPmax * (a0 * a0 + a1 * a1 + a2 * a2));
+            lam[d] = lc;
+            lam[3 + d] = lv;
+        }
+    }
+    // wave reduction (max, NaN-propagating)
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        double v = lam[q];
+        for (int off = 32; off > 0; off >>= 1) {
+            double o = __shfl_xor(v, off, 64);
+            v = (o > v || o != o) ? o : v;
+        }
+        if ((tid & 63) == 0) s_red[q][tid >> 6] = v;
+    }
+    int anybad = __syncthreads_or(bad);
+    if (tid == 0) {
+        double m[6];
+        for (int q = 0; q < 6; q++) {
+            double v = s_red[q][0];
+            for (int w = 1; w < nw; w++) v = (s_red[q][w] > v || s_red[q][w] != s_red[q][w]) ? s_red[q][w] : v;
+            m[q] = v;
+        }
+        dtc[e] = CFLScale * 2.0 / (m[0] + m[1] + m[2]);
+        dtv[e] = DFLScale * 4.0 / (m[3] + m[4] + m[5]);
+        if (anybad) atomicOr(err, 1);
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_dt_reduce(int nElems, const double* __restrict__ dtc,
+                                                    const double* __restrict__ dtv, int* __restrict__ err,
+                                                    double* __restrict__ out)
+{
+    __shared__ double s[2][16];
+    __shared__ int s_nan[2];
+    const int tid = threadIdx.x;
+    if (tid < 2) s_nan[tid] = 0;
+    __syncthreads();
+    double a = 1e300, b = 1e300;
+    int na = 0, nb = 0;
+    for (int x = tid; x < nElems; x += 1024) {
+        double va = dtc[x], vb = dtv[x];
+        if (va != va) na = 1; else a = fmin(a, va);
+        if (vb != vb) nb = 1; else b = fmin(b, vb);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmin(a, __shfl_xor(a, off, 64));
+        b = fmin(b, __shfl_xor(b, off, 64));
+    }
+    if (na) atomicOr(&s_nan[0], 1);
+    if (nb) atomicOr(&s_nan[1], 1);
+    if ((tid & 63) == 0) {
+        s[0][tid >> 6] = a;
+        s[1][tid >> 6] = b;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < 16; w++) {
+            a = fmin(a, s[0][w]);
+            b = fmin(b, s[1][w]);
+        }
+        out[0] = a;
+        out[1] = b;
+        out[2] = a < b ? a : b;
+        int ev = err[0] ? 1 : (s_nan[0] ? 2 : (s_nan[1] ? 3 : 0));
+        out[3] = double(ev);
+    }
+}
+
+}  // namespace galx

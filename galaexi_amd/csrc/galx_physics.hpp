@@ -1,0 +1,293 @@
+// galx_physics.hpp -- pointwise compressible Navier-Stokes physics for the gfx950 kernels.
+//
+// Templated on the arithmetic type T (double for the FP64 form, float for the mixed form).  Each routine
+// restates the reference routine cited next to it and keeps its operation order, so that the FP64 form
+// agrees with the CPU restatement to rounding.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace galx {
+
+enum { RIEMANN_LF = 0, RIEMANN_ROE = 1, RIEMANN_ROEEF = 3 };
+
+template <typename T>
+struct Gas {
+    T kappa, R, mu, lambda;
+};
+
+__device__ __forceinline__ double gsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float gsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double gabs(double x) { return fabs(x); }
+__device__ __forceinline__ float gabs(float x) { return fabsf(x); }
+__device__ __forceinline__ double gmax(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float gmax(float a, float b) { return fmaxf(a, b); }
+
+// ConsToPrim (src/equations/navierstokes/idealgas/eos.f90:212-239): prim = (rho,u,v,w,p,T)
+template <typename T>
+__device__ __forceinline__ void cons_to_prim(const T* c, T* p, const Gas<T>& g)
+{
+    T sRho = T(1) / c[0];
+    p[0] = c[0];
+    p[1] = c[1] * sRho;
+    p[2] = c[2] * sRho;
+    p[3] = c[3] * sRho;
+    p[4] = (g.kappa - T(1)) * (c[4] - T(0.5) * (c[1] * p[1] + c[2] * p[2] + c[3] * p[3]));
+    p[5] = p[4] * sRho / g.R;
+}
+
+// extended state in the normal frame (eos.h:119-135): DENS, MOM(3), ENER, SRHO, VEL(3), PRES
+template <typename T>
+struct Ext {
+    T dens, mom[3], ener, srho, vel[3], pres;
+};
+
+template <typename T>
+__device__ __forceinline__ void rotate_state(const T* U, const T* P, const T* nv, const T* t1, const T* t2, Ext<T>& X)
+{
+    X.dens = U[0];
+    X.srho = T(1) / X.dens;
+    X.ener = U[4];
+    X.pres = P[4];
+    X.vel[0] = P[1] * nv[0] + P[2] * nv[1] + P[3] * nv[2];
+    X.vel[1] = P[1] * t1[0] + P[2] * t1[1] + P[3] * t1[2];
+    X.vel[2] = P[1] * t2[0] + P[2] * t2[1] + P[3] * t2[2];
+    X.mom[0] = X.dens * X.vel[0];
+    X.mom[1] = X.dens * X.vel[1];
+    X.mom[2] = X.dens * X.vel[2];
+}
+
+// EvalEulerFlux1D_fast (src/equations/navierstokes/flux.f90:1123-1143)
+template <typename T>
+__device__ __forceinline__ void euler_flux_1d(const Ext<T>& X, T* F)
+{
+    F[0] = X.mom[0];
+    F[1] = X.mom[0] * X.vel[0] + X.pres;
+    F[2] = X.mom[0] * X.vel[1];
+    F[3] = X.mom[0] * X.vel[2];
+    F[4] = (X.ener + X.pres) * X.vel[0];
+}
+
+// SplitSurfaceFluxPI (src/equations/navierstokes/splitflux.f90:735-761)
+template <typename T>
+__device__ __forceinline__ void split_surface_flux_pi(const Ext<T>& L, const Ext<T>& R, T* F)
+{
+    T HL = (L.ener + L.pres) / L.dens;
+    T HR = (R.ener + R.pres) / R.dens;
+    T sr = L.dens + R.dens, su = L.vel[0] + R.vel[0];
+    F[0] = T(0.25) * sr * su;
+    F[1] = T(0.125) * sr * (su * su) + T(0.5) * (L.pres + R.pres);
+    F[2] = T(0.125) * sr * su * (L.vel[1] + R.vel[1]);
+    F[3] = T(0.125) * sr * su * (L.vel[2] + R.vel[2]);
+    F[4] = T(0.125) * sr * (HL + HR) * su;
+}
+
+__device__ __forceinline__ double ext_cons(const Ext<double>& X, int v)
+{
+    return v == 0 ? X.dens : (v == 4 ? X.ener : X.mom[v - 1]);
+}
+__device__ __forceinline__ float ext_cons(const Ext<float>& X, int v)
+{
+    return v == 0 ? X.dens : (v == 4 ? X.ener : X.mom[v - 1]);
+}
+
+// Riemann_LF / Riemann_Roe / Riemann_RoeEntropyFix (src/equations/navierstokes/riemann.f90:707-989);
+// SPLIT selects the SPLIT_DG branch (PI surface flux as the central part).
+template <typename T, bool SPLIT>
+__device__ __forceinline__ void riemann_solver(int solver, const Ext<T>& L, const Ext<T>& R, const T* FL, const T* FR,
+                                               T kappa, T* F)
+{
+    if (solver == RIEMANN_LF) {
+        T cL = gsqrt(kappa * L.pres * L.srho), cR = gsqrt(kappa * R.pres * R.srho);
+        T lmax = gmax(gabs(R.vel[0]), gabs(L.vel[0])) + gmax(cL, cR);
+        if (!SPLIT) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) F[v] = T(0.5) * ((FL[v] + FR[v]) - lmax * (ext_cons(R, v) - ext_cons(L, v)));
+        } else {
+            split_surface_flux_pi(L, R, F);
+#pragma unroll
+            for (int v = 0; v < 5; v++) F[v] = F[v] - T(0.5) * lmax * (ext_cons(R, v) - ext_cons(L, v));
+        }
+        return;
+    }
+    T HL = (L.ener + L.pres) * L.srho, HR = (R.ener + R.pres) * R.srho;
+    T sqL = gsqrt(L.dens), sqR = gsqrt(R.dens);
+    T ss = T(1) / (sqL + sqR);
+    T rv0 = (sqR * R.vel[0] + sqL * L.vel[0]) * ss;
+    T rv1 = (sqR * R.vel[1] + sqL * L.vel[1]) * ss;
+    T rv2 = (sqR * R.vel[2] + sqL * L.vel[2]) * ss;
+    T absVel = rv0 * rv0 + rv1 * rv1 + rv2 * rv2;
+    T RoeH = (sqR * HR + sqL * HL) * ss;
+    T Roec = gsqrt((kappa - T(1)) * (RoeH - T(0.5) * absVel));
+    T a[5] = {rv0 - Roec, rv0, rv0, rv0, rv0 + Roec};
+    T al[5], aa[5];
+    if (solver == RIEMANN_ROE) {
+        T dU[6];
+#pragma unroll
+        for (int v = 0; v < 5; v++) dU[v] = ext_cons(R, v) - ext_cons(L, v);
+        dU[5] = dU[4] - (dU[2] - rv1 * dU[0]) * rv1 - (dU[3] - rv2 * dU[0]) * rv2;
+        al[2] = dU[2] - rv1 * dU[0];
+        al[3] = dU[3] - rv2 * dU[0];
+        al[1] = (kappa - T(1)) / (Roec * Roec) * (dU[0] * (RoeH - rv0 * rv0) - dU[5] + rv0 * dU[1]);
+        al[0] = T(0.5) / Roec * (dU[0] * (rv0 + Roec) - dU[1] - Roec * al[1]);
+        al[4] = dU[0] - al[0] - al[1];
+#pragma unroll
+        for (int v = 0; v < 5; v++) aa[v] = gabs(a[v]);
+    } else {
+        T cL = gsqrt(kappa * L.pres * L.srho), cR = gsqrt(kappa * R.pres * R.srho);
+        T RoeDens = gsqrt(L.dens * R.dens);
+        T d0 = R.dens - L.dens, d1 = R.vel[0] - L.vel[0], d2 = R.vel[1] - L.vel[1], d3 = R.vel[2] - L.vel[2],
+          d4 = R.pres - L.pres;
+        T tmp = T(0.5) / (Roec * Roec);
+        al[0] = tmp * (d4 - RoeDens * Roec * d1);
+        al[1] = d0 - d4 * T(2) * tmp;
+        al[2] = RoeDens * d2;
+        al[3] = RoeDens * d3;
+        al[4] = tmp * (d4 + RoeDens * Roec * d1);
+        T aL[5] = {L.vel[0] - cL, L.vel[0], L.vel[0], L.vel[0], L.vel[0] + cL};
+        T aR[5] = {R.vel[0] - cR, R.vel[0], R.vel[0], R.vel[0], R.vel[0] + cR};
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            T da = gmax(T(0), gmax(a[v] - aL[v], aR[v] - a[v]));
+            aa[v] = (gabs(a[v]) < da) ? T(0.5) * (a[v] * a[v] / da + da) : gabs(a[v]);
+        }
+    }
+    T r1[5] = {T(1), a[0], rv1, rv2, RoeH - rv0 * Roec};
+    T r2[5] = {T(1), rv0, rv1, rv2, T(0.5) * absVel};
+    T r3[5] = {T(0), T(0), T(1), T(0), rv1};
+    T r4[5] = {T(0), T(0), T(0), T(1), rv2};
+    T r5[5] = {T(1), a[4], rv1, rv2, RoeH + rv0 * Roec};
+    if (!SPLIT) {
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            F[v] = T(0.5) * ((FL[v] + FR[v]) - al[0] * aa[0] * r1[v] - al[1] * aa[1] * r2[v] - al[2] * aa[2] * r3[v] -
+                             al[3] * aa[3] * r4[v] - al[4] * aa[4] * r5[v]);
+    } else {
+        split_surface_flux_pi(L, R, F);
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            F[v] = F[v] - T(0.5) * (al[0] * aa[0] * r1[v] + al[1] * aa[1] * r2[v] + al[2] * aa[2] * r3[v] +
+                                    al[3] * aa[3] * r4[v] + al[4] * aa[4] * r5[v]);
+    }
+}
+
+// Riemann driver (src/equations/navierstokes/riemann.f90:216-289): rotate, solve, rotate back.
+template <typename T, bool SPLIT>
+__device__ __forceinline__ void riemann_point(int solver, const T* UL, const T* UR, const T* PL, const T* PR, const T* nv,
+                                              const T* t1, const T* t2, T kappa, T* Fout)
+{
+    Ext<T> L, R;
+    rotate_state(UL, PL, nv, t1, t2, L);
+    rotate_state(UR, PR, nv, t1, t2, R);
+    T FL[5] = {0, 0, 0, 0, 0}, FR[5] = {0, 0, 0, 0, 0}, F[5];
+    if (!SPLIT) {
+        euler_flux_1d(L, FL);
+        euler_flux_1d(R, FR);
+    }
+    riemann_solver<T, SPLIT>(solver, L, R, FL, FR, kappa, F);
+    Fout[0] = F[0];
+#pragma unroll
+    for (int d = 0; d < 3; d++) Fout[1 + d] = nv[d] * F[1] + t1[d] * F[2] + t2[d] * F[3];
+    Fout[4] = F[4];
+}
+
+// gradients of the lifted variables in the order used here: g[4*dir + c], dir in {x,y,z}, c in {u,v,w,T}
+// Shear stress and heat flux terms shared by EvalNormalDiffFlux3D (flux.f90:779-847) and
+// EvalTransformed(Euler)DiffFlux3D (flux.f90:509-700).
+template <typename T>
+struct Stress {
+    T txx, tyy, tzz, txy, txz, tyz, qx, qy, qz;
+};
+
+template <typename T>
+__device__ __forceinline__ void stress(const T* P, const T* g, const Gas<T>& gas, Stress<T>& s)
+{
+    const T s23 = T(2.0 / 3.0), s43 = T(4.0 / 3.0);
+    const T* gx = g;
+    const T* gy = g + 4;
+    const T* gz = g + 8;
+    T mu = gas.mu;
+    s.txx = mu * (s43 * gx[0] - s23 * gy[1] - s23 * gz[2]);
+    s.tyy = mu * (-s23 * gx[0] + s43 * gy[1] - s23 * gz[2]);
+    s.tzz = mu * (-s23 * gx[0] - s23 * gy[1] + s43 * gz[2]);
+    s.txy = mu * (gy[0] + gx[1]);
+    s.txz = mu * (gz[0] + gx[2]);
+    s.tyz = mu * (gz[1] + gy[2]);
+    s.qx = s.txx * P[1] + s.txy * P[2] + s.txz * P[3] + gas.lambda * gx[3];
+    s.qy = s.txy * P[1] + s.tyy * P[2] + s.tyz * P[3] + gas.lambda * gy[3];
+    s.qz = s.txz * P[1] + s.tyz * P[2] + s.tzz * P[3] + gas.lambda * gz[3];
+}
+
+// EvalNormalDiffFlux3D (flux.f90:779-847)
+template <typename T>
+__device__ __forceinline__ void normal_diff_flux(const T* P, const T* g, const T* nv, const Gas<T>& gas, T* f)
+{
+    const T s23 = T(2.0 / 3.0), s43 = T(4.0 / 3.0);
+    const T* gx = g;
+    const T* gy = g + 4;
+    const T* gz = g + 8;
+    T mu = gas.mu;
+    T txx = mu * (s43 * gx[0] - s23 * gy[1] - s23 * gz[2]);
+    T tyy = mu * (-s23 * gx[0] + s43 * gy[1] - s23 * gz[2]);
+    T tzz = mu * (-s23 * gx[0] - s23 * gy[1] + s43 * gz[2]);
+    T txy = mu * (gy[0] + gx[1]);
+    T txz = mu * (gz[0] + gx[2]);
+    T tyz = mu * (gz[1] + gy[2]);
+    f[0] = T(0);
+    f[1] = -nv[0] * txx - nv[1] * txy - nv[2] * txz;
+    f[2] = -nv[0] * txy - nv[1] * tyy - nv[2] * tyz;
+    f[3] = -nv[0] * txz - nv[1] * tyz - nv[2] * tzz;
+    f[4] = -nv[0] * (txx * P[1] + txy * P[2] + txz * P[3] + gas.lambda * gx[3]) -
+           nv[1] * (txy * P[1] + tyy * P[2] + tyz * P[3] + gas.lambda * gy[3]) -
+           nv[2] * (txz * P[1] + tyz * P[2] + tzz * P[3] + gas.lambda * gz[3]);
+}
+
+// contravariant flux in one direction with metric m: EvalTransformedEulerDiffFlux3D (flux.f90:509-614) when
+// EULER, else EvalTransformedDiffFlux3D (flux.f90:619-700)
+template <typename T, bool EULER>
+__device__ __forceinline__ void transformed_flux(const T* U, const T* P, const Stress<T>& s, const T* m, T* r)
+{
+    if (EULER) {
+        T Ep = (U[4] + P[4]) / U[0];
+        T Mmom = m[0] * U[1] + m[1] * U[2] + m[2] * U[3];
+        r[0] = Mmom;
+        r[1] = Mmom * P[1] + m[0] * P[4] - m[0] * s.txx - m[1] * s.txy - m[2] * s.txz;
+        r[2] = Mmom * P[2] + m[1] * P[4] - m[0] * s.txy - m[1] * s.tyy - m[2] * s.tyz;
+        r[3] = Mmom * P[3] + m[2] * P[4] - m[0] * s.txz - m[1] * s.tyz - m[2] * s.tzz;
+        r[4] = Mmom * Ep - m[0] * s.qx - m[1] * s.qy - m[2] * s.qz;
+    } else {
+        r[0] = T(0);
+        r[1] = -m[0] * s.txx - m[1] * s.txy - m[2] * s.txz;
+        r[2] = -m[0] * s.txy - m[1] * s.tyy - m[2] * s.tyz;
+        r[3] = -m[0] * s.txz - m[1] * s.tyz - m[2] * s.tzz;
+        r[4] = -m[0] * s.qx - m[1] * s.qy - m[2] * s.qz;
+    }
+}
+
+// SplitVolumeFluxPI (src/equations/navierstokes/splitflux.f90:669-730) with the node-wise (rho,u,v,w,p,H)
+// precomputed; H = (E+p)/rho as in the reference.
+template <typename T>
+__device__ __forceinline__ void split_volume_flux_pi(T rR, T uR, T vR, T wR, T pR, T HR, T r, T u, T v, T w, T p, T H,
+                                                     const T* MR, const T* M, T* Flux)
+{
+    T sr = rR + r, su = uR + u, sv = vR + v, sw = wR + w, sp = pR + p, sH = HR + H;
+    T ft0 = T(0.5) * sr * su;
+    T ft1 = T(0.25) * sr * (su * su) + sp;
+    T ft2 = T(0.25) * sr * su * sv;
+    T ft3 = T(0.25) * sr * su * sw;
+    T ft4 = T(0.25) * sr * su * sH;
+    T gt0 = T(0.5) * sr * sv;
+    T gt2 = T(0.25) * sr * (sv * sv) + sp;
+    T gt3 = T(0.25) * sr * sv * sw;
+    T gt4 = T(0.25) * sr * sv * sH;
+    T ht0 = T(0.5) * sr * sw;
+    T ht3 = T(0.25) * sr * (sw * sw) + sp;
+    T ht4 = T(0.25) * sr * sw * sH;
+    T a = T(0.5) * (MR[0] + M[0]), c = T(0.5) * (MR[2] + M[2]), b = T(0.5) * (MR[1] + M[1]);
+    Flux[0] = a * ft0 + c * ht0 + b * gt0;
+    Flux[1] = a * ft1 + c * ft3 + b * ft2;
+    Flux[2] = a * ft2 + c * gt3 + b * gt2;
+    Flux[3] = a * ft3 + c * ht3 + b * gt3;
+    Flux[4] = a * ft4 + c * ht4 + b * gt4;
+}
+
+}  // namespace galx

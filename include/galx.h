@@ -1,0 +1,126 @@
+/*
+ * galx.h -- C ABI of the MI355X (gfx950) DGSEM right-hand-side engine.
+ *
+ * Drop-in for the GALAEXI hot path: DGTimeDerivative_weakForm (src/dg/dg.f90:255-425), the low-storage RK
+ * step TimeStepByLSERKW2 (src/timedisc/timestep.f90:49-120) and CALCTIMESTEP
+ * (src/equations/navierstokes/calctimestep.f90:98-186).  The reference passes data through Fortran module
+ * globals; here the host hands over the same arrays, in the same Fortran layouts, once per mesh and state.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative GALX_E* code on failure; galx_last_error() gives a
+ *    message.  Nothing throws across the ABI.
+ *  - Host arrays use the reference layouts (column-major, 0-based offsets, 1-based IDs inside connectivity):
+ *      U(5,0:N,0:N,0:N,nElems), Metrics_{f,g,h}Tilde(3,0:N,0:N,0:N,nElems), sJ(0:N,0:N,0:N,nElems),
+ *      NormVec/TangVec1/TangVec2(3,0:N,0:N,nSides), SurfElem(0:N,0:N,nSides),
+ *      ElemToSide(3,6,nElems), SideToElem(5,nSides), S2V2(2,0:N,0:N,0:4,6)
+ *    (src/dg/dg.f90:106-152, src/mesh/mesh_vars.f90:62-70, src/flexi.h:98-115, src/mesh/mappings.f90:75-235).
+ *  - A context is single-threaded.  All work goes to one HIP stream owned by the context; every call returns
+ *    after that stream has finished unless stated otherwise.
+ *  - Only periodic conforming meshes (nBCSides = 0, no mortars), Gauss-Lobatto nodes, constant viscosity,
+ *    BR1 lifting in strong form.
+ */
+#ifndef GALX_H
+#define GALX_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GALX_OK 0
+#define GALX_E_ARG (-1)      /* invalid argument or unsupported configuration */
+#define GALX_E_STATE (-2)    /* call order violated (e.g. rhs before set_mesh) */
+#define GALX_E_HIP (-3)      /* HIP runtime error */
+#define GALX_E_NOMEM (-4)    /* device allocation failed */
+#define GALX_E_MESH (-5)     /* inconsistent connectivity */
+
+/* volume integral form: VolInt_weakForm (src/dg/volint.f90:129-188) or SPLIT_DG with the Pirozzoli flux
+ * (src/dg/volint.f90:211-353, src/equations/navierstokes/splitflux.f90:669-761) */
+#define GALX_FORM_WEAK 0
+#define GALX_FORM_SPLIT_PI 1
+/* Riemann solver, numbered as the reference's -DRIEMANN (src/CMakeLists.txt:100-107) */
+#define GALX_RIEMANN_LF 0
+#define GALX_RIEMANN_ROE 1
+#define GALX_RIEMANN_ROEEF 3
+/* arithmetic: plain FP64, or mixed (FP32 fluxes, lifting, volume/surface integrals and traces; FP64 U,
+ * metrics, Jacobian, LSERK registers and time-step reduction) */
+#define GALX_PREC_FP64 0
+#define GALX_PREC_MIXED 1
+
+typedef struct galx_config {
+    int N;           /* polynomial degree (PP_N); supported set: galx_supported_degrees() */
+    int form;        /* GALX_FORM_* */
+    int riemann;     /* GALX_RIEMANN_* */
+    int precision;   /* GALX_PREC_* */
+    double kappa;    /* EOS_Vars(EOS_KAPPA) */
+    double R;        /* EOS_Vars(EOS_R) */
+    double Pr;       /* EOS_Vars(EOS_PR) */
+    double mu0;      /* EOS_Vars(EOS_MU0) */
+    double CFLScale; /* ini CFLScale, before the alpha(N)/(2N+1) scaling (src/timedisc/timedisc_func.f90:480-550) */
+    double DFLScale; /* ini DFLScale */
+    int device;      /* HIP device ordinal */
+} galx_config;
+
+typedef struct galx_ctx galx_ctx;
+
+/* Library version string and the degrees N the kernels are compiled for (bit N set). */
+const char *galx_version(void);
+unsigned galx_supported_degrees(int precision);
+const char *galx_last_error(void);
+
+/* InitDG/InitEquation/InitTimeDisc counterpart: allocate a context (src/dg/dg.f90:67-174). */
+int galx_create(const galx_config *cfg, galx_ctx **out);
+int galx_destroy(galx_ctx *ctx);
+
+/* InitMesh counterpart (src/mesh/mesh.f90:307-451): copy connectivity and geometry (host pointers, reference
+ * layouts) into the device layout.  Validates SideToElem/ElemToSide consistency. */
+int galx_set_mesh(galx_ctx *ctx, int nElems, int nSides, const int *ElemToSide, const int *SideToElem,
+                  const int *S2V2, const double *Metrics_fTilde, const double *Metrics_gTilde,
+                  const double *Metrics_hTilde, const double *sJ, const double *NormVec, const double *TangVec1,
+                  const double *TangVec2, const double *SurfElem);
+
+/* d_U = U and U = d_U (src/dg/dg.f90:166-169, src/timedisc/timedisc_func.f90:348-349), host arrays. */
+int galx_set_state(galx_ctx *ctx, const double *U);
+int galx_get_state(galx_ctx *ctx, double *U);
+/* Same with device pointers in the reference layout (zero host copy). */
+int galx_set_state_device(galx_ctx *ctx, const double *dU);
+int galx_get_state_device(galx_ctx *ctx, double *dU);
+
+/* DGTimeDerivative_weakForm(t) (src/dg/dg.f90:255-425): Ut from the current state.  t is accepted for
+ * interface parity; the periodic operator has no explicit time dependence. */
+int galx_rhs(galx_ctx *ctx, double t);
+/* Ut of the last galx_rhs in the reference layout (host / device pointer). */
+int galx_get_ut(galx_ctx *ctx, double *Ut);
+int galx_get_ut_device(galx_ctx *ctx, double *dUt);
+
+/* TimeStepByLSERKW2(t) with Carpenter RK4-5 (src/timedisc/timestep.f90:49-120): advances the state by dt.
+ * Ut_tmp is kept on the device between calls (zeroed by galx_set_state). */
+int galx_lserk_step(galx_ctx *ctx, double t, double dt);
+/* nSteps steps with a fixed dt, without host synchronisation between steps. */
+int galx_lserk_steps(galx_ctx *ctx, double t, double dt, int nSteps);
+
+/* CALCTIMESTEP(errType) (src/equations/navierstokes/calctimestep.f90:98-186), FP64 reduction.
+ * errType: 0 ok, 1 non-positive or NaN density, 2 convective / 3 viscous time step is NaN. */
+int galx_calc_dt(galx_ctx *ctx, double *dt, int *errType);
+
+/* Unit-level entry points mirroring the reference routines exercised by its unit tests (host pointers in the
+ * reference layouts; work runs on the GPU of `device`).  GL nodes, non-EXACT_MM. */
+/* ProlongToFaceCons -> EvalElemFaceGL_Device (src/interpolation/prolongtoface.t90:116-162,268-344) */
+int galx_prolong_to_face(int device, int nVar, int N, int nElems, int nSides, const int *SideToElem,
+                         const int *S2V2, const double *Uvol, double *Uface_master, double *Uface_slave);
+/* SurfIntCons -> SurfInt_Device (src/dg/surfint.t90:237-290,352-586); Ut is updated in place. */
+int galx_surf_int(int device, int nVar, int N, int nElems, int nSides, const int *SideToElem, const int *S2V2,
+                  const double *Flux_master, const double *Flux_slave, double L_HatMinus0, double L_HatPlusN,
+                  double *Ut);
+
+/* Kernel timers: when enabled, every kernel launch of the context is bracketed with HIP events on the
+ * context stream; galx_timer_read returns the accumulated milliseconds and launch count of a kernel family
+ * ("trace", "lift", "rhs", "dt"). */
+int galx_timers_enable(galx_ctx *ctx, int on);
+int galx_timer_read(galx_ctx *ctx, const char *name, double *ms, long *count);
+int galx_timers_reset(galx_ctx *ctx);
+/* Device-memory footprint of the context in bytes. */
+int galx_memory_bytes(galx_ctx *ctx, long long *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
