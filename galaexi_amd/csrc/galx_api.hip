@@ -141,17 +141,34 @@ __global__ void k_planes_to_ref(long nDOF, int n3, const double* __restrict__ sr
     for (int v = 0; v < 5; v++) dst[5 * x + v] = src[(e * 5 + v) * n3 + node];
 }
 
+// sum of the pressure over all nodes (element planes), for the pressure offset of the mixed form
+__global__ void __launch_bounds__(256) k_pressure_sum(long nDOF, int n3, double kappa, const double* __restrict__ U,
+                                                      double* __restrict__ out)
+{
+    __shared__ double s[4];
+    long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double p = 0.0;
+    if (x < nDOF) {
+        long e = x / n3;
+        int node = (int)(x - e * n3);
+        const double* u = U + e * 5 * n3 + node;
+        double rho = u[0], sRho = 1.0 / rho;
+        double m1 = u[n3], m2 = u[2 * n3], m3 = u[3 * n3];
+        p = (kappa - 1.0) * (u[4 * n3] - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * sRho);
+    }
+    for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = p;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, s[0] + s[1] + s[2] + s[3]);
+}
+
 // ----------------------------------------------------------------------------------------------------------
 // kernel sets per (N, precision, form)
 // ----------------------------------------------------------------------------------------------------------
-struct GasD {
-    double kappa, R, mu, lambda;
-};
-
 struct KernelSet {
     bool ok = false;
     int block = 0;
-    void (*trace)(hipStream_t, int, const double*, Mesh, void*) = nullptr;
+    void (*trace)(hipStream_t, int, const double*, Mesh, void*, GasD) = nullptr;
     void (*lift)(hipStream_t, int, const double*, Mesh, const void*, void*, const void*, GasD, double, double) = nullptr;
     void (*rhs)(hipStream_t, int, double*, Mesh, const void*, const void*, void*, double*, double*, const void*, GasD,
                 int, double, double, Stage) = nullptr;
@@ -159,31 +176,35 @@ struct KernelSet {
                double*, int*) = nullptr;
 };
 
-template <typename T>
-static Gas<T> to_gas(GasD g)
-{
-    return Gas<T>{T(g.kappa), T(g.R), T(g.mu), T(g.lambda)};
-}
-
 template <int N, typename T, bool SPLIT>
 struct Launch {
     using D = Dm<N>;
-    static void trace(hipStream_t s, int nE, const double* U, Mesh m, void* utr)
+    static void trace(hipStream_t s, int nE, const double* U, Mesh m, void* utr, GasD g)
     {
-        hipLaunchKernelGGL((k_trace<N, T>), dim3(nE), dim3(D::block), 0, s, U, m, (T*)utr);
+        hipLaunchKernelGGL((k_trace<N, T>), dim3(nE), dim3(D::block), 0, s, U, m, (T*)utr, g);
     }
     static void lift(hipStream_t s, int nE, const double* U, Mesh m, const void* utr, void* gtr, const void* mats,
                      GasD g, double LHm, double LHp)
     {
         hipLaunchKernelGGL((k_lift<N, T, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr, (T*)gtr,
-                           (const T*)mats, to_gas<T>(g), T(LHm), T(LHp));
+                           (const T*)mats, g, T(LHm), T(LHp));
+    }
+    template <int RIE>
+    static void rhs_r(hipStream_t s, int nE, double* U, Mesh m, const void* utr, const void* gtr, void* utr_next,
+                      double* Ut, double* K, const void* mats, GasD g, double LHm, double LHp, Stage st)
+    {
+        hipLaunchKernelGGL((k_rhs<N, T, T, SPLIT, RIE>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr,
+                           (const T*)gtr, (T*)utr_next, Ut, K, (const T*)mats, g, T(LHm), T(LHp), st);
     }
     static void rhs(hipStream_t s, int nE, double* U, Mesh m, const void* utr, const void* gtr, void* utr_next,
                     double* Ut, double* K, const void* mats, GasD g, int riemann, double LHm, double LHp, Stage st)
     {
-        hipLaunchKernelGGL((k_rhs<N, T, T, SPLIT>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr,
-                           (const T*)gtr, (T*)utr_next, Ut, K, (const T*)mats, to_gas<T>(g), riemann, T(LHm), T(LHp),
-                           st);
+        if (riemann == GALX_RIEMANN_LF)
+            rhs_r<RIEMANN_LF>(s, nE, U, m, utr, gtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
+        else if (riemann == GALX_RIEMANN_ROE)
+            rhs_r<RIEMANN_ROE>(s, nE, U, m, utr, gtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
+        else
+            rhs_r<RIEMANN_ROEEF>(s, nE, U, m, utr, gtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
     }
     static void dt(hipStream_t s, int nE, const double* U, const double* geo, double kappa, double mu0, double KPmax,
                    double CFL, double DFL, double* dtc, double* dtv, int* err)
@@ -246,6 +267,7 @@ struct galx_ctx {
     KernelSet ks;
     Basis basis;
     double lambda = 0.0;
+    double pref = 0.0;  // pressure offset of the mixed form (mean initial pressure); 0 in FP64
     // device buffers
     double *dU = nullptr, *dK = nullptr, *dUt = nullptr, *dScratch = nullptr;
     double *dGeo = nullptr, *dFgeo = nullptr;
@@ -314,7 +336,7 @@ static void flush_timers(galx_ctx* c)
 }
 
 static Mesh mesh_of(galx_ctx* c) { return Mesh{c->dGeo, c->dFgeo, c->dConn, c->dInv}; }
-static GasD gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda}; }
+static GasD gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, c->pref}; }
 
 extern "C" {
 
@@ -521,9 +543,22 @@ static int after_state(galx_ctx* c)
 {
     HIPCHK(hipMemsetAsync(c->dK, 0, 5L * c->nElems * c->n3 * sizeof(double), c->stream));
     c->cur = 0;
+    c->pref = 0.0;
+    if (c->cfg.precision == GALX_PREC_MIXED) {
+        long nDOF = (long)c->nElems * c->n3;
+        HIPCHK(hipMemsetAsync(c->dDtOut, 0, sizeof(double), c->stream));
+        k_pressure_sum<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->cfg.kappa,
+                                                                                         c->dU, c->dDtOut);
+        HIPCHK(hipGetLastError());
+        double sum = 0.0;
+        HIPCHK(hipMemcpyAsync(&sum, c->dDtOut, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->pref = sum / (double)nDOF;
+        if (!(c->pref == c->pref)) c->pref = 0.0;
+    }
     {
         TimerScope ts(c, "trace");
-        c->ks.trace(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[0]);
+        c->ks.trace(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[0], gas_of(c));
     }
     HIPCHK(hipGetLastError());
     c->stateSet = true;

@@ -16,6 +16,13 @@
 #pragma once
 #include "galx_physics.hpp"
 
+#ifndef GALX_UNROLL_L
+#define GALX_UNROLL_L 2
+#endif
+#ifndef GALX_UNROLL_SPLIT
+#define GALX_UNROLL_SPLIT 1
+#endif
+
 namespace galx {
 
 template <int N>
@@ -53,6 +60,17 @@ __device__ __forceinline__ bool on_face(int ls, int i, int j, int k, int& ab)
     case 5: ab = j + n * k; return i == 0;
     default: ab = i + n * j; return k == N;
     }
+}
+
+// gas constants and the pressure offset in FP64; each kernel derives its Gas<T>
+struct GasD {
+    double kappa, R, mu, lambda, pref;
+};
+
+template <typename T>
+__device__ __forceinline__ Gas<T> make_gas(const GasD& d)
+{
+    return Gas<T>{T(d.kappa), T(d.R), T(d.mu), T(d.lambda), T(d.pref), T(d.kappa * d.pref / (d.kappa - 1.0))};
 }
 
 struct Mesh {
@@ -112,10 +130,10 @@ __device__ __forceinline__ void lift_gradient(int e, int tid, const Mesh& mesh, 
         int flip = c & 7;
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
         const TT* nb = utr + ((side * 2 + (flip ? 0 : 1)) * 5) * n2 + pq;
-        T uc[5], pr[6];
+        T tr[5], uc[5], pr[6];
 #pragma unroll
-        for (int v = 0; v < 5; v++) uc[v] = T(nb[v * n2]);
-        cons_to_prim(uc, pr, gas);
+        for (int v = 0; v < 5; v++) tr[v] = T(nb[v * n2]);
+        trace_state(tr, gas, uc, pr);
         T Lnb[4] = {pr[1], pr[2], pr[3], pr[5]};
         const double* fg = mesh.fgeo + side * 10 * n2 + pq;
         T se = T(fg[9 * n2]);
@@ -134,7 +152,7 @@ __device__ __forceinline__ void lift_gradient(int e, int tid, const Mesh& mesh, 
     if (tid < n3) {
         int i = tid % n, j = (tid / n) % n, k = tid / n2;
         T gxi[4] = {0, 0, 0, 0}, get[4] = {0, 0, 0, 0}, gze[4] = {0, 0, 0, 0};
-#pragma unroll
+#pragma unroll GALX_UNROLL_L
         for (int l = 0; l <= N; l++) {
             T di = M.DT[l + n * i], dj = M.DT[l + n * j], dk = M.DT[l + n * k];
 #pragma unroll
@@ -179,18 +197,41 @@ __device__ __forceinline__ void lift_gradient(int e, int tid, const Mesh& mesh, 
     }
 }
 
+// volume state of node tid: ConsToPrim (eos.f90:212-239) in FP64, then shifted by the pressure offset and
+// rounded to T; H = (E+p)/rho also from the FP64 values
 template <int N, typename T>
-__device__ __forceinline__ void load_state(int e, int tid, const double* __restrict__ U, double* ud, T* uc, T* pr,
-                                           const Gas<T>& gas)
+__device__ __forceinline__ void load_state(int e, int tid, const double* __restrict__ U, const GasD& gd, T* uc, T* pr,
+                                           T& H)
 {
     constexpr int n3 = Dm<N>::n3;
     const double* u = U + (long)e * 5 * n3 + tid;
+    double c[5];
 #pragma unroll
-    for (int v = 0; v < 5; v++) {
-        ud[v] = u[v * n3];
-        uc[v] = T(ud[v]);
-    }
-    cons_to_prim(uc, pr, gas);
+    for (int v = 0; v < 5; v++) c[v] = u[v * n3];
+    double sRho = 1.0 / c[0];
+    double vx = c[1] * sRho, vy = c[2] * sRho, vz = c[3] * sRho;
+    double p = (gd.kappa - 1.0) * (c[4] - 0.5 * (c[1] * vx + c[2] * vy + c[3] * vz));
+    uc[0] = T(c[0]);
+    uc[1] = T(c[1]);
+    uc[2] = T(c[2]);
+    uc[3] = T(c[3]);
+    uc[4] = T(c[4] - gd.pref / (gd.kappa - 1.0));
+    pr[0] = T(c[0]);
+    pr[1] = T(vx);
+    pr[2] = T(vy);
+    pr[3] = T(vz);
+    pr[4] = T(p - gd.pref);
+    pr[5] = T(p * sRho / gd.R);
+    H = T((c[4] + p) / c[0]);
+}
+
+template <typename TT>
+__device__ __forceinline__ void store_trace(TT* o, int n2, const double* c, const GasD& gd)
+{
+    double tr[5];
+    trace_record(c, gd.kappa, gd.pref, tr);
+#pragma unroll
+    for (int v = 0; v < 5; v++) o[v * n2] = TT(tr[v]);
 }
 
 // ---------------------------------------------------------------------------------------------------------
@@ -198,7 +239,8 @@ __device__ __forceinline__ void load_state(int e, int tid, const double* __restr
 // element-wise form: each element writes its own slot of each of its 6 sides).
 // ---------------------------------------------------------------------------------------------------------
 template <int N, typename TT>
-__global__ void __launch_bounds__(Dm<N>::block) k_trace(const double* __restrict__ U, Mesh mesh, TT* __restrict__ utr)
+__global__ void __launch_bounds__(Dm<N>::block)
+    k_trace(const double* __restrict__ U, Mesh mesh, TT* __restrict__ utr, GasD gd)
 {
     using D = Dm<N>;
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
@@ -212,8 +254,10 @@ __global__ void __launch_bounds__(Dm<N>::block) k_trace(const double* __restrict
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
         TT* o = utr + ((side * 2 + (flip ? 1 : 0)) * 5) * n2 + pq;
         const double* u = U + (long)e * 5 * n3 + node;
+        double cv[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) o[v * n2] = TT(u[v * n3]);
+        for (int v = 0; v < 5; v++) cv[v] = u[v * n3];
+        store_trace(o, n2, cv, gd);
     }
 }
 
@@ -223,7 +267,7 @@ __global__ void __launch_bounds__(Dm<N>::block) k_trace(const double* __restrict
 template <int N, typename T, typename TT>
 __global__ void __launch_bounds__(Dm<N>::block)
     k_lift(const double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, TT* __restrict__ gtr,
-           const T* __restrict__ mats, Gas<T> gas, T LHm, T LHp)
+           const T* __restrict__ mats, GasD gd, T LHm, T LHp)
 {
     using D = Dm<N>;
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
@@ -234,10 +278,10 @@ __global__ void __launch_bounds__(Dm<N>::block)
     T* s_lift = s_buf;
     T* s_lflux = s_buf + 4 * n3;
     load_mats<N, T>(M, mats, tid, D::block);
-    double ud[5];
-    T uc[5], pr[6];
+    const Gas<T> gas = make_gas<T>(gd);
+    T uc[5], pr[6], H;
     if (tid < n3) {
-        load_state<N, T>(e, tid, U, ud, uc, pr, gas);
+        load_state<N, T>(e, tid, U, gd, uc, pr, H);
         s_lift[tid] = pr[1];
         s_lift[n3 + tid] = pr[2];
         s_lift[2 * n3 + tid] = pr[3];
@@ -280,66 +324,105 @@ __global__ void __launch_bounds__(Dm<N>::block)
 //   else : VolInt_weakForm with EvalTransformedEulerDiffFlux3D (src/dg/volint.f90:129-188)
 //   FillFlux: Riemann + ViscousFlux + SurfElem (src/dg/fillflux.f90:45-172), SurfInt_Device (surfint.t90:352-586)
 //   Ut = -sJ*Ut (src/dg/dg.f90:413,423)
+// LDS: region W (work: lifting, volume fluxes, split-form partner data, face results, new traces) and region G
+// (own gradients on the six faces, written after the lifting and read by the face-flux tasks), so that the
+// gradient registers die before the volume phases.
 // ---------------------------------------------------------------------------------------------------------
-template <int N, typename T, typename TT, bool SPLIT>
-__global__ void __launch_bounds__(Dm<N>::block)
+template <int N, typename T, typename TT, bool SPLIT, int RIEMANN>
+__device__ __forceinline__ void face_flux(const T* tro, const T* Go, const T* trn, const T* Gn,
+                                          const double* __restrict__ fg, int n2, int flip, const Gas<T>& gas, T* F)
+{
+    T Uo[5], Un[5], Po[6], Pn[6];
+    trace_state(tro, gas, Uo, Po);
+    trace_state(trn, gas, Un, Pn);
+    T nv[3] = {T(fg[0]), T(fg[n2]), T(fg[2 * n2])};
+    // ViscousFlux (riemann.f90:638-700): 0.5*(F_v(master) + F_v(slave)).n, formed before the inviscid part
+    T fvm[5], fvs[5];
+    normal_diff_flux(flip ? Pn : Po, flip ? Gn : Go, nv, gas, fvm);
+    normal_diff_flux(flip ? Po : Pn, flip ? Go : Gn, nv, gas, fvs);
+    T t1[3] = {T(fg[3 * n2]), T(fg[4 * n2]), T(fg[5 * n2])};
+    T t2[3] = {T(fg[6 * n2]), T(fg[7 * n2]), T(fg[8 * n2])};
+    const T *UL = flip ? Un : Uo, *UR = flip ? Uo : Un, *PL = flip ? Pn : Po, *PR = flip ? Po : Pn;
+    riemann_point<T, SPLIT>(RIEMANN, UL, UR, PL, PR, nv, t1, t2, gas, F);
+    T se = T(fg[9 * n2]);
+#pragma unroll
+    for (int v = 0; v < 5; v++) F[v] = ((F[v] + T(0.5) * fvm[v]) + T(0.5) * fvs[v]) * se;
+}
+
+// minimum waves per SIMD for k_rhs: 4 lets two 512-thread workgroups share a CU at N=7 (the register
+// allocation is capped at 128); the heavier entropy-fixed Roe solver keeps the compiler's choice.
+template <int RIEMANN>
+struct RhsWaves {
+    static constexpr int value = RIEMANN == RIEMANN_ROEEF ? 1 : 4;
+};
+
+template <int N, typename T, typename TT, bool SPLIT, int RIEMANN>
+__global__ void __launch_bounds__(Dm<N>::block, RhsWaves<RIEMANN>::value)
     k_rhs(double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, const TT* __restrict__ gtr,
           TT* __restrict__ utr_next, double* __restrict__ Ut, double* __restrict__ Kreg, const T* __restrict__ mats,
-          Gas<T> gas, int riemann, T LHm, T LHp, Stage st)
+          GasD gd, T LHm, T LHp, Stage st)
 {
     using D = Dm<N>;
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
-    constexpr int lds_a = 4 * n3 + 6 * 7 * n2;
-    constexpr int lds_c = SPLIT ? 9 * n3 : 5 * n3;
-    constexpr int lds_d = 6 * 17 * n2;
-    constexpr int lds_ac = lds_a > lds_c ? lds_a : lds_c;
-    __shared__ T s_buf[lds_ac > lds_d ? lds_ac : lds_d];
+    constexpr int w_a = 4 * n3 + 6 * 7 * n2;          // lifting
+    constexpr int w_c = SPLIT ? 9 * n3 : 5 * n3;      // volume fluxes / split-form partner data
+    constexpr int w_d = 6 * 5 * n2;                   // face results, new traces
+    constexpr int w_ac = w_a > w_c ? w_a : w_c;
+    constexpr int lds_w = w_ac > w_d ? w_ac : w_d;
+    __shared__ T s_w[lds_w];
+    __shared__ T s_g[6 * 12 * n2];
     __shared__ Mats<N, T> M;
     const int e = blockIdx.x, tid = threadIdx.x;
     const bool active = tid < n3;
     const int i = tid % n, j = (tid / n) % n, k = tid / n2;
     load_mats<N, T>(M, mats, tid, D::block);
-    double ud[5];
-    T uc[5], pr[6], g[12];
-    T* s_lift = s_buf;
+    const Gas<T> gas = make_gas<T>(gd);
+    T uc[5], pr[6], Hme = T(0);
     if (active) {
-        load_state<N, T>(e, tid, U, ud, uc, pr, gas);
-        s_lift[tid] = pr[1];
-        s_lift[n3 + tid] = pr[2];
-        s_lift[2 * n3 + tid] = pr[3];
-        s_lift[3 * n3 + tid] = pr[5];
+        load_state<N, T>(e, tid, U, gd, uc, pr, Hme);
+        s_w[tid] = pr[1];
+        s_w[n3 + tid] = pr[2];
+        s_w[2 * n3 + tid] = pr[3];
+        s_w[3 * n3 + tid] = pr[5];
     }
     __syncthreads();
-    lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_lift, s_buf + 4 * n3, gas, LHm, LHp, g);
+    Stress<T> s;
+    {
+        T g[12];
+        lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3, gas, LHm, LHp, g);
+        if (active) {
+            stress(pr, g, gas, s);
+#pragma unroll
+            for (int ls = 1; ls <= 6; ls++) {
+                int ab;
+                if (!on_face<N>(ls, i, j, k, ab)) continue;
+                T* o = s_g + (ls - 1) * 12 * n2 + ab;
+#pragma unroll
+                for (int q = 0; q < 12; q++) o[q * n2] = g[q];
+            }
+        }
+    }
     __syncthreads();
 
     // ---- viscous (split) or full (weak) transformed fluxes, weak-form D_Hat_T contraction ----
     T ut[5] = {0, 0, 0, 0, 0};
-    T ja[9];
-    double sJd = 0.0;
-    Stress<T> s;
-    if (active) {
-        const double* ge = mesh.geo + (long)e * 10 * n3 + tid;
-#pragma unroll
-        for (int q = 0; q < 9; q++) ja[q] = T(ge[q * n3]);
-        sJd = ge[9 * n3];
-        stress(pr, g, gas, s);
-    }
-    T* s_f = s_buf;  // [5][n3]
-#pragma unroll
+    const double* ge = mesh.geo + (long)e * 10 * n3 + tid;
+    T* s_f = s_w;  // [5][n3]
+#pragma unroll 1
     for (int d = 0; d < 3; d++) {
         if (active) {
+            T m[3] = {T(ge[(3 * d) * n3]), T(ge[(3 * d + 1) * n3]), T(ge[(3 * d + 2) * n3])};
             T f[5];
-            transformed_flux<T, !SPLIT>(uc, pr, s, ja + 3 * d, f);
+            transformed_flux<T, !SPLIT>(uc, pr, s, m, gas.hshift, f);
 #pragma unroll
-            for (int v = 0; v < 5; v++) s_f[v * n3 + tid] = f[v];
+            for (int v = (SPLIT ? 1 : 0); v < 5; v++) s_f[v * n3 + tid] = f[v];
         }
         __syncthreads();
         if (active) {
             int idx = d == 0 ? i : (d == 1 ? j : k);
             int stride = d == 0 ? 1 : (d == 1 ? n : n2);
             int base = tid - idx * stride;
-#pragma unroll
+#pragma unroll GALX_UNROLL_L
             for (int l = 0; l <= N; l++) {
                 T dh = M.DhT[l + n * idx];
 #pragma unroll
@@ -351,31 +434,34 @@ __global__ void __launch_bounds__(Dm<N>::block)
 
     // ---- split-form flux differencing (VolInt_splitForm_Kernel) ----
     if (SPLIT) {
-        T* s_v = s_buf;           // [6][n3] rho,u,v,w,p,H
-        T* s_m = s_buf + 6 * n3;  // [3][n3] metric of the current direction
+        T* s_v = s_w;           // [6][n3] rho,u,v,w,p,H
+        T* s_m = s_w + 6 * n3;  // [3][n3] metric of the current direction
         if (active) {
             s_v[tid] = pr[0];
             s_v[n3 + tid] = pr[1];
             s_v[2 * n3 + tid] = pr[2];
             s_v[3 * n3 + tid] = pr[3];
             s_v[4 * n3 + tid] = pr[4];
-            s_v[5 * n3 + tid] = (uc[4] + pr[4]) / uc[0];
+            s_v[5 * n3 + tid] = Hme;
         }
         T tmp[5] = {0, 0, 0, 0, 0};
-        T Hme = active ? (uc[4] + pr[4]) / uc[0] : T(0);
-#pragma unroll
+#pragma unroll 1
         for (int d = 0; d < 3; d++) {
+            T mme[3] = {0, 0, 0};
             if (active) {
-                s_m[tid] = ja[3 * d];
-                s_m[n3 + tid] = ja[3 * d + 1];
-                s_m[2 * n3 + tid] = ja[3 * d + 2];
+                mme[0] = T(ge[(3 * d) * n3]);
+                mme[1] = T(ge[(3 * d + 1) * n3]);
+                mme[2] = T(ge[(3 * d + 2) * n3]);
+                s_m[tid] = mme[0];
+                s_m[n3 + tid] = mme[1];
+                s_m[2 * n3 + tid] = mme[2];
             }
             __syncthreads();
             if (active) {
                 int idx = d == 0 ? i : (d == 1 ? j : k);
                 int stride = d == 0 ? 1 : (d == 1 ? n : n2);
                 int base = tid - idx * stride;
-#pragma unroll
+#pragma unroll GALX_UNROLL_SPLIT
                 for (int l = 0; l <= N; l++) {
                     if (l == idx) continue;  // DVolSurf diagonal vanishes for Gauss-Lobatto (src/dg/dg.f90:229-237)
                     int o = base + l * stride;
@@ -383,7 +469,7 @@ __global__ void __launch_bounds__(Dm<N>::block)
                     T F[5];
                     split_volume_flux_pi<T>(pr[0], pr[1], pr[2], pr[3], pr[4], Hme, s_v[o], s_v[n3 + o],
                                             s_v[2 * n3 + o], s_v[3 * n3 + o], s_v[4 * n3 + o], s_v[5 * n3 + o],
-                                            ja + 3 * d, Mo, F);
+                                            mme, Mo, F);
                     T dv = M.DVS[l + n * idx];
 #pragma unroll
                     for (int v = 0; v < 5; v++) tmp[v] = tmp[v] + dv * F[v];
@@ -396,36 +482,27 @@ __global__ void __launch_bounds__(Dm<N>::block)
     }
 
     // ---- face fluxes (FillFlux) and surface integral (SurfInt_Device) ----
-    T* s_face = s_buf;  // [6][17][n2]: own cons(5), own grad(12); later results [6][5][n2]
-    if (active) {
-#pragma unroll
-        for (int ls = 1; ls <= 6; ls++) {
-            int ab;
-            if (!on_face<N>(ls, i, j, k, ab)) continue;
-            T* o = s_face + (ls - 1) * 17 * n2 + ab;
-#pragma unroll
-            for (int v = 0; v < 5; v++) o[v * n2] = uc[v];
-#pragma unroll
-            for (int q = 0; q < 12; q++) o[(5 + q) * n2] = g[q];
-        }
-    }
-    __syncthreads();
-    T res[D::ft][5];
-#pragma unroll
-    for (int r = 0; r < D::ft; r++) {
-        int t = tid + r * D::block;
-        if (t >= D::nf) continue;
-        int ls = t / n2 + 1, ab = t - (ls - 1) * n2;
+    T* s_res = s_w;  // [6][5][n2]
+    for (int t = tid; t < D::nf; t += D::block) {
+        int ls = t / n2 + 1, ab = t - (ls - 1) * n2, a = ab % n, b = ab / n;
+        int node = face_vol_node<N>(ls, a, b);
         int c = mesh.conn[e * 6 + ls - 1];
         long side = c >> 3;
         int flip = c & 7;
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
-        const T* so = s_face + (ls - 1) * 17 * n2 + ab;
-        T Uo[5], Po[6], Go[12], Un[5], Pn[6], Gn[12];
+        T Uo[5], Go[12], Un[5], Gn[12];
+        {
+            const double* uo = U + (long)e * 5 * n3 + node;
+            double c[5], tr[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) Uo[v] = so[v * n2];
+            for (int v = 0; v < 5; v++) c[v] = uo[v * n3];
+            trace_record(c, gd.kappa, gd.pref, tr);
 #pragma unroll
-        for (int q = 0; q < 12; q++) Go[q] = so[(5 + q) * n2];
+            for (int v = 0; v < 5; v++) Uo[v] = T(TT(tr[v]));  // exactly the record the neighbour reads
+        }
+        const T* so = s_g + (ls - 1) * 12 * n2 + ab;
+#pragma unroll
+        for (int q = 0; q < 12; q++) Go[q] = so[q * n2];
         const long nslot = side * 2 + (flip ? 0 : 1);
         const TT* un = utr + nslot * 5 * n2 + pq;
         const TT* gn = gtr + nslot * 12 * n2 + pq;
@@ -433,35 +510,11 @@ __global__ void __launch_bounds__(Dm<N>::block)
         for (int v = 0; v < 5; v++) Un[v] = T(un[v * n2]);
 #pragma unroll
         for (int q = 0; q < 12; q++) Gn[q] = T(gn[q * n2]);
-        cons_to_prim(Uo, Po, gas);
-        cons_to_prim(Un, Pn, gas);
-        const double* fg = mesh.fgeo + side * 10 * n2 + pq;
-        T nv[3] = {T(fg[0]), T(fg[n2]), T(fg[2 * n2])};
-        T t1[3] = {T(fg[3 * n2]), T(fg[4 * n2]), T(fg[5 * n2])};
-        T t2[3] = {T(fg[6 * n2]), T(fg[7 * n2]), T(fg[8 * n2])};
-        T se = T(fg[9 * n2]);
-        const T *UL = flip ? Un : Uo, *UR = flip ? Uo : Un, *PL = flip ? Pn : Po, *PR = flip ? Po : Pn;
-        const T *GL = flip ? Gn : Go, *GR = flip ? Go : Gn;
-        T F[5], fv[5];
-        riemann_point<T, SPLIT>(riemann, UL, UR, PL, PR, nv, t1, t2, gas.kappa, F);
-        normal_diff_flux(PL, GL, nv, gas, fv);
-#pragma unroll
-        for (int v = 0; v < 5; v++) F[v] = F[v] + T(0.5) * fv[v];
-        normal_diff_flux(PR, GR, nv, gas, fv);
-#pragma unroll
-        for (int v = 0; v < 5; v++) F[v] = F[v] + T(0.5) * fv[v];
+        T F[5];
+        face_flux<N, T, TT, SPLIT, RIEMANN>(Uo, Go, Un, Gn, mesh.fgeo + side * 10 * n2 + pq, n2, flip, gas, F);
         T sg = flip ? T(-1) : T(1);
 #pragma unroll
-        for (int v = 0; v < 5; v++) res[r][v] = sg * (F[v] * se);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < D::ft; r++) {
-        int t = tid + r * D::block;
-        if (t >= D::nf) continue;
-        int ls = t / n2 + 1, ab = t - (ls - 1) * n2;
-#pragma unroll
-        for (int v = 0; v < 5; v++) s_face[(ls - 1) * 5 * n2 + v * n2 + ab] = res[r][v];
+        for (int v = 0; v < 5; v++) s_res[(ls - 1) * 5 * n2 + v * n2 + ab] = sg * F[v];
     }
     __syncthreads();
     double unew[5];
@@ -473,8 +526,9 @@ __global__ void __launch_bounds__(Dm<N>::block)
             if (!on_face<N>(ls, i, j, k, ab)) continue;
             T LH = (ls == 1 || ls == 2 || ls == 5) ? LHm : LHp;
 #pragma unroll
-            for (int v = 0; v < 5; v++) loc[v] = loc[v] + s_face[(ls - 1) * 5 * n2 + v * n2 + ab] * LH;
+            for (int v = 0; v < 5; v++) loc[v] = loc[v] + s_res[(ls - 1) * 5 * n2 + v * n2 + ab] * LH;
         }
+        double sJd = ge[9 * n3];
         double* ut_out = (st.mode == 0 ? Ut : Kreg) + (long)e * 5 * n3 + tid;
         double* uo = U + (long)e * 5 * n3 + tid;
 #pragma unroll
@@ -486,7 +540,7 @@ __global__ void __launch_bounds__(Dm<N>::block)
             } else {
                 double kk = ut_out[v * n3] * st.mRKA + utd;
                 ut_out[v * n3] = kk;
-                unew[v] = ud[v] + kk * st.bdt;
+                unew[v] = uo[v * n3] + kk * st.bdt;
                 uo[v * n3] = unew[v];
             }
         }
@@ -494,14 +548,16 @@ __global__ void __launch_bounds__(Dm<N>::block)
     if (st.mode == 0) return;
     // ---- next U traces ----
     __syncthreads();
-    T* s_u = s_buf;  // [6][5][n2] new U on the faces, rounded to the trace type
+    T* s_u = s_w;  // [6][5][n2] trace records of the new U, rounded to the trace type
     if (active) {
+        double tr[5];
+        trace_record(unew, gd.kappa, gd.pref, tr);
 #pragma unroll
         for (int ls = 1; ls <= 6; ls++) {
             int ab;
             if (!on_face<N>(ls, i, j, k, ab)) continue;
 #pragma unroll
-            for (int v = 0; v < 5; v++) s_u[(ls - 1) * 5 * n2 + v * n2 + ab] = T(TT(unew[v]));
+            for (int v = 0; v < 5; v++) s_u[(ls - 1) * 5 * n2 + v * n2 + ab] = T(TT(tr[v]));
         }
     }
     __syncthreads();

@@ -3,6 +3,13 @@
 // Templated on the arithmetic type T (double for the FP64 form, float for the mixed form).  Each routine
 // restates the reference routine cited next to it and keeps its operation order, so that the FP64 form
 // agrees with the CPU restatement to rounding.
+//
+// Pressure offset: states carry p' = p - pref and E' = E - pref/(kappa-1) for one constant pref per context.
+// A constant pressure has zero discrete divergence on free-stream preserving meshes, so p' can replace p in
+// every additive pressure term of the momentum fluxes; the full values are restored where they enter
+// nonlinearly (sound speed: p'+pref, enthalpy: E'+p'+hshift with hshift = kappa*pref/(kappa-1)).  The FP64
+// form uses pref = 0 (reference arithmetic); the mixed form uses the mean initial pressure so that the FP32
+// values hold the pressure fluctuation at full relative precision.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -12,7 +19,7 @@ enum { RIEMANN_LF = 0, RIEMANN_ROE = 1, RIEMANN_ROEEF = 3 };
 
 template <typename T>
 struct Gas {
-    T kappa, R, mu, lambda;
+    T kappa, R, mu, lambda, pref, hshift;
 };
 
 __device__ __forceinline__ double gsqrt(double x) { return sqrt(x); }
@@ -33,6 +40,38 @@ __device__ __forceinline__ void cons_to_prim(const T* c, T* p, const Gas<T>& g)
     p[3] = c[3] * sRho;
     p[4] = (g.kappa - T(1)) * (c[4] - T(0.5) * (c[1] * p[1] + c[2] * p[2] + c[3] * p[3]));
     p[5] = p[4] * sRho / g.R;
+}
+
+// face state from a trace record (rho, rho*u, rho*v, rho*w, p'): Uc = (rho, rho*u, rho*v, rho*w, E'),
+// P = (rho, u, v, w, p', T); ConsToPrim (eos.f90:212-239) with the pressure taken from the record
+template <typename T>
+__device__ __forceinline__ void trace_state(const T* tr, const Gas<T>& g, T* Uc, T* P)
+{
+    T sRho = T(1) / tr[0];
+    P[0] = tr[0];
+    P[1] = tr[1] * sRho;
+    P[2] = tr[2] * sRho;
+    P[3] = tr[3] * sRho;
+    P[4] = tr[4];
+    P[5] = (tr[4] + g.pref) * sRho / g.R;
+    Uc[0] = tr[0];
+    Uc[1] = tr[1];
+    Uc[2] = tr[2];
+    Uc[3] = tr[3];
+    Uc[4] = tr[4] / (g.kappa - T(1)) + T(0.5) * (tr[1] * P[1] + tr[2] * P[2] + tr[3] * P[3]);
+}
+
+// trace record of a conservative FP64 state (ConsToPrim pressure, shifted by pref)
+__device__ __forceinline__ void trace_record(const double* c, double kappa, double pref, double* tr)
+{
+    double sRho = 1.0 / c[0];
+    double u = c[1] * sRho, v = c[2] * sRho, w = c[3] * sRho;
+    double p = (kappa - 1.0) * (c[4] - 0.5 * (c[1] * u + c[2] * v + c[3] * w));
+    tr[0] = c[0];
+    tr[1] = c[1];
+    tr[2] = c[2];
+    tr[3] = c[3];
+    tr[4] = p - pref;
 }
 
 // extended state in the normal frame (eos.h:119-135): DENS, MOM(3), ENER, SRHO, VEL(3), PRES
@@ -58,21 +97,21 @@ __device__ __forceinline__ void rotate_state(const T* U, const T* P, const T* nv
 
 // EvalEulerFlux1D_fast (src/equations/navierstokes/flux.f90:1123-1143)
 template <typename T>
-__device__ __forceinline__ void euler_flux_1d(const Ext<T>& X, T* F)
+__device__ __forceinline__ void euler_flux_1d(const Ext<T>& X, T hshift, T* F)
 {
     F[0] = X.mom[0];
     F[1] = X.mom[0] * X.vel[0] + X.pres;
     F[2] = X.mom[0] * X.vel[1];
     F[3] = X.mom[0] * X.vel[2];
-    F[4] = (X.ener + X.pres) * X.vel[0];
+    F[4] = (X.ener + X.pres + hshift) * X.vel[0];
 }
 
 // SplitSurfaceFluxPI (src/equations/navierstokes/splitflux.f90:735-761)
 template <typename T>
-__device__ __forceinline__ void split_surface_flux_pi(const Ext<T>& L, const Ext<T>& R, T* F)
+__device__ __forceinline__ void split_surface_flux_pi(const Ext<T>& L, const Ext<T>& R, T hshift, T* F)
 {
-    T HL = (L.ener + L.pres) / L.dens;
-    T HR = (R.ener + R.pres) / R.dens;
+    T HL = (L.ener + L.pres + hshift) / L.dens;
+    T HR = (R.ener + R.pres + hshift) / R.dens;
     T sr = L.dens + R.dens, su = L.vel[0] + R.vel[0];
     F[0] = T(0.25) * sr * su;
     F[1] = T(0.125) * sr * (su * su) + T(0.5) * (L.pres + R.pres);
@@ -94,22 +133,23 @@ __device__ __forceinline__ float ext_cons(const Ext<float>& X, int v)
 // SPLIT selects the SPLIT_DG branch (PI surface flux as the central part).
 template <typename T, bool SPLIT>
 __device__ __forceinline__ void riemann_solver(int solver, const Ext<T>& L, const Ext<T>& R, const T* FL, const T* FR,
-                                               T kappa, T* F)
+                                               const Gas<T>& g, T* F)
 {
+    const T kappa = g.kappa, hshift = g.hshift;
     if (solver == RIEMANN_LF) {
-        T cL = gsqrt(kappa * L.pres * L.srho), cR = gsqrt(kappa * R.pres * R.srho);
+        T cL = gsqrt(kappa * (L.pres + g.pref) * L.srho), cR = gsqrt(kappa * (R.pres + g.pref) * R.srho);
         T lmax = gmax(gabs(R.vel[0]), gabs(L.vel[0])) + gmax(cL, cR);
         if (!SPLIT) {
 #pragma unroll
             for (int v = 0; v < 5; v++) F[v] = T(0.5) * ((FL[v] + FR[v]) - lmax * (ext_cons(R, v) - ext_cons(L, v)));
         } else {
-            split_surface_flux_pi(L, R, F);
+            split_surface_flux_pi(L, R, hshift, F);
 #pragma unroll
             for (int v = 0; v < 5; v++) F[v] = F[v] - T(0.5) * lmax * (ext_cons(R, v) - ext_cons(L, v));
         }
         return;
     }
-    T HL = (L.ener + L.pres) * L.srho, HR = (R.ener + R.pres) * R.srho;
+    T HL = (L.ener + L.pres + hshift) * L.srho, HR = (R.ener + R.pres + hshift) * R.srho;
     T sqL = gsqrt(L.dens), sqR = gsqrt(R.dens);
     T ss = T(1) / (sqL + sqR);
     T rv0 = (sqR * R.vel[0] + sqL * L.vel[0]) * ss;
@@ -133,7 +173,7 @@ __device__ __forceinline__ void riemann_solver(int solver, const Ext<T>& L, cons
 #pragma unroll
         for (int v = 0; v < 5; v++) aa[v] = gabs(a[v]);
     } else {
-        T cL = gsqrt(kappa * L.pres * L.srho), cR = gsqrt(kappa * R.pres * R.srho);
+        T cL = gsqrt(kappa * (L.pres + g.pref) * L.srho), cR = gsqrt(kappa * (R.pres + g.pref) * R.srho);
         T RoeDens = gsqrt(L.dens * R.dens);
         T d0 = R.dens - L.dens, d1 = R.vel[0] - L.vel[0], d2 = R.vel[1] - L.vel[1], d3 = R.vel[2] - L.vel[2],
           d4 = R.pres - L.pres;
@@ -162,7 +202,7 @@ __device__ __forceinline__ void riemann_solver(int solver, const Ext<T>& L, cons
             F[v] = T(0.5) * ((FL[v] + FR[v]) - al[0] * aa[0] * r1[v] - al[1] * aa[1] * r2[v] - al[2] * aa[2] * r3[v] -
                              al[3] * aa[3] * r4[v] - al[4] * aa[4] * r5[v]);
     } else {
-        split_surface_flux_pi(L, R, F);
+        split_surface_flux_pi(L, R, hshift, F);
 #pragma unroll
         for (int v = 0; v < 5; v++)
             F[v] = F[v] - T(0.5) * (al[0] * aa[0] * r1[v] + al[1] * aa[1] * r2[v] + al[2] * aa[2] * r3[v] +
@@ -173,17 +213,17 @@ __device__ __forceinline__ void riemann_solver(int solver, const Ext<T>& L, cons
 // Riemann driver (src/equations/navierstokes/riemann.f90:216-289): rotate, solve, rotate back.
 template <typename T, bool SPLIT>
 __device__ __forceinline__ void riemann_point(int solver, const T* UL, const T* UR, const T* PL, const T* PR, const T* nv,
-                                              const T* t1, const T* t2, T kappa, T* Fout)
+                                              const T* t1, const T* t2, const Gas<T>& g, T* Fout)
 {
     Ext<T> L, R;
     rotate_state(UL, PL, nv, t1, t2, L);
     rotate_state(UR, PR, nv, t1, t2, R);
     T FL[5] = {0, 0, 0, 0, 0}, FR[5] = {0, 0, 0, 0, 0}, F[5];
     if (!SPLIT) {
-        euler_flux_1d(L, FL);
-        euler_flux_1d(R, FR);
+        euler_flux_1d(L, g.hshift, FL);
+        euler_flux_1d(R, g.hshift, FR);
     }
-    riemann_solver<T, SPLIT>(solver, L, R, FL, FR, kappa, F);
+    riemann_solver<T, SPLIT>(solver, L, R, FL, FR, g, F);
     Fout[0] = F[0];
 #pragma unroll
     for (int d = 0; d < 3; d++) Fout[1 + d] = nv[d] * F[1] + t1[d] * F[2] + t2[d] * F[3];
@@ -244,10 +284,10 @@ __device__ __forceinline__ void normal_diff_flux(const T* P, const T* g, const T
 // contravariant flux in one direction with metric m: EvalTransformedEulerDiffFlux3D (flux.f90:509-614) when
 // EULER, else EvalTransformedDiffFlux3D (flux.f90:619-700)
 template <typename T, bool EULER>
-__device__ __forceinline__ void transformed_flux(const T* U, const T* P, const Stress<T>& s, const T* m, T* r)
+__device__ __forceinline__ void transformed_flux(const T* U, const T* P, const Stress<T>& s, const T* m, T hshift, T* r)
 {
     if (EULER) {
-        T Ep = (U[4] + P[4]) / U[0];
+        T Ep = (U[4] + P[4] + hshift) / U[0];
         T Mmom = m[0] * U[1] + m[1] * U[2] + m[2] * U[3];
         r[0] = Mmom;
         r[1] = Mmom * P[1] + m[0] * P[4] - m[0] * s.txx - m[1] * s.txy - m[2] * s.txz;
@@ -264,30 +304,24 @@ __device__ __forceinline__ void transformed_flux(const T* U, const T* P, const S
 }
 
 // SplitVolumeFluxPI (src/equations/navierstokes/splitflux.f90:669-730) with the node-wise (rho,u,v,w,p,H)
-// precomputed; H = (E+p)/rho as in the reference.
+// precomputed (H = (E+p)/rho as in the reference).  The reference forms the three Cartesian two-point fluxes and
+// contracts them with 0.5*(M_ref+M); the same sum is regrouped around the contravariant mass flux
+//   q = 1/4 {rho}' (0.5*(M_ref+M) . {u}')   with {x}' = x_ref + x:
+//   F = [2q, q{u}' + 0.5(M_ref+M)_1 {p}', q{v}' + 0.5(M_ref+M)_2 {p}', q{w}' + 0.5(M_ref+M)_3 {p}', q{H}']
+// which is algebraically identical and takes ~25 instead of ~70 operations per pair.  p may be shifted by a
+// constant (its contribution to the discrete divergence vanishes with free-stream preserving metrics).
 template <typename T>
 __device__ __forceinline__ void split_volume_flux_pi(T rR, T uR, T vR, T wR, T pR, T HR, T r, T u, T v, T w, T p, T H,
                                                      const T* MR, const T* M, T* Flux)
 {
     T sr = rR + r, su = uR + u, sv = vR + v, sw = wR + w, sp = pR + p, sH = HR + H;
-    T ft0 = T(0.5) * sr * su;
-    T ft1 = T(0.25) * sr * (su * su) + sp;
-    T ft2 = T(0.25) * sr * su * sv;
-    T ft3 = T(0.25) * sr * su * sw;
-    T ft4 = T(0.25) * sr * su * sH;
-    T gt0 = T(0.5) * sr * sv;
-    T gt2 = T(0.25) * sr * (sv * sv) + sp;
-    T gt3 = T(0.25) * sr * sv * sw;
-    T gt4 = T(0.25) * sr * sv * sH;
-    T ht0 = T(0.5) * sr * sw;
-    T ht3 = T(0.25) * sr * (sw * sw) + sp;
-    T ht4 = T(0.25) * sr * sw * sH;
-    T a = T(0.5) * (MR[0] + M[0]), c = T(0.5) * (MR[2] + M[2]), b = T(0.5) * (MR[1] + M[1]);
-    Flux[0] = a * ft0 + c * ht0 + b * gt0;
-    Flux[1] = a * ft1 + c * ft3 + b * ft2;
-    Flux[2] = a * ft2 + c * gt3 + b * gt2;
-    Flux[3] = a * ft3 + c * ht3 + b * gt3;
-    Flux[4] = a * ft4 + c * ht4 + b * gt4;
+    T m0 = MR[0] + M[0], m1 = MR[1] + M[1], m2 = MR[2] + M[2];
+    T q = T(0.125) * sr * (m0 * su + m1 * sv + m2 * sw);
+    Flux[0] = T(2) * q;
+    Flux[1] = q * su + (T(0.5) * m0) * sp;
+    Flux[2] = q * sv + (T(0.5) * m1) * sp;
+    Flux[3] = q * sw + (T(0.5) * m2) * sp;
+    Flux[4] = q * sH;
 }
 
 }  // namespace galx

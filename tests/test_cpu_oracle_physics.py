@@ -1,0 +1,130 @@
+"""The FP64 oracle against properties the mathematics fixes: discrete conservation, element-frame
+equivariance, isentropic-vortex convergence order (regressioncheck/checks/convtest rule: rate >= 0.8 of
+N+1), fourth-order LSERK coefficients, the closed-form time step and the errType path."""
+import numpy as np
+import pytest
+
+import oracle as O
+from galaexi_amd import mesh as M
+from galaexi_amd import testcases as T
+
+
+def vol_weights(N):
+    w = O.dg_basis(N)["wGP"]
+    return w[:, None, None] * w[None, :, None] * w[None, None, :]
+
+
+FORMS = [(True, O.RIEMANN_LF), (True, O.RIEMANN_ROEEF), (False, O.RIEMANN_ROE), (False, O.RIEMANN_LF)]
+
+
+@pytest.mark.parametrize("split,riemann", FORMS)
+def test_conservation_curved(split, riemann):
+    """Periodic mesh: sum_nodes w J Ut = 0 for every conservative variable."""
+    m = M.box_mesh((3, 2, 2), 3, lo=(-1, -1, -1), hi=(1, 1, 1), deform_amp=0.05, deform_phase=(0.2, 0.4, 0.6),
+                   rotations="random", seed=2)
+    U = T.smooth_random(m.Elem_xGP, (-1, -1, -1), (1, 1, 1), seed=3)
+    Ut = O.Operator(m, split=split, riemann=riemann, mu0=2e-2, R=1.0).rhs(U)
+    W = vol_weights(3)
+    tot = np.einsum("ijk,vijke,ijke->v", W, Ut, 1.0 / m.sJ)
+    scale = np.einsum("ijk,vijke,ijke->v", W, np.abs(Ut), 1.0 / m.sJ)
+    assert np.all(np.abs(tot) <= 1e-13 * scale)
+
+
+@pytest.mark.parametrize("split,riemann", FORMS[:3])
+def test_element_frame_equivariance(split, riemann):
+    """Relabelling element frames (all 24 rotations, every flip) maps Ut node-for-node."""
+    N = 3
+    m0 = M.box_mesh((3, 3, 2), N, lo=(-2, -2, -2), hi=(2, 2, 2))
+    m1 = M.box_mesh((3, 3, 2), N, lo=(-2, -2, -2), hi=(2, 2, 2), rotations="random", seed=7)
+    kw = dict(center=(0.3, -0.2, 0.1), amplitude=0.8, axis=(0.3, 0.5, 0.8))
+    Ut0 = O.Operator(m0, split=split, riemann=riemann, mu0=2e-2, R=1.0).rhs(T.shu_vortex(m0.Elem_xGP, **kw))
+    Ut1 = O.Operator(m1, split=split, riemann=riemann, mu0=2e-2, R=1.0).rhs(T.shu_vortex(m1.Elem_xGP, **kw))
+    n = N + 1
+    c0 = m0.Elem_xGP.reshape(3, n ** 3, -1, order="F").mean(axis=1).T
+    c1 = m1.Elem_xGP.reshape(3, n ** 3, -1, order="F").mean(axis=1).T
+    err = 0.0
+    for e1 in range(m1.nElems):
+        e0 = int(np.argmin(np.sum((c0 - c1[e1]) ** 2, axis=1)))
+        x0 = m0.Elem_xGP[..., e0].reshape(3, -1, order="F").T
+        x1 = m1.Elem_xGP[..., e1].reshape(3, -1, order="F").T
+        perm = [int(np.argmin(np.sum((x0 - p) ** 2, axis=1))) for p in x1]
+        u0 = Ut0[..., e0].reshape(5, -1, order="F")[:, perm]
+        u1 = Ut1[..., e1].reshape(5, -1, order="F")
+        err = max(err, np.abs(u0 - u1).max())
+    assert err < 1e-12 * np.abs(Ut0).max()
+
+
+@pytest.mark.parametrize("split,riemann", [(False, O.RIEMANN_ROE), (True, O.RIEMANN_LF)])
+def test_isentropic_vortex_convergence(split, riemann):
+    """h-convergence of the Shu vortex (exactfunc.f90:482-503) at N=3, inviscid: rate >= 0.8*(N+1)."""
+    N, tend = 3, 0.5
+    errs = []
+    for ne in (4, 8):
+        m = M.box_mesh((ne, ne, 1), N, lo=(-5, -5, -5), hi=(5, 5, 5))
+        kw = dict(center=(0.3, -0.2, 0.0), amplitude=0.8, axis=(0, 0, 1))
+        U = np.asfortranarray(T.shu_vortex(m.Elem_xGP, **kw))
+        op = O.Operator(m, split=split, riemann=riemann, kappa=1.4, R=1.0, Pr=0.72, mu0=0.0)
+        dt = op.calc_dt(U, 0.5, 0.5)[0]
+        nst = int(np.ceil(tend / dt))
+        dt = tend / nst
+        K = np.zeros_like(U, order="F")
+        for s in range(nst):
+            op.lserk_step(U, K, s * dt, dt)
+        Ue = T.shu_vortex(m.Elem_xGP, t=tend, **kw)
+        W, J = vol_weights(N), 1.0 / m.sJ
+        errs.append(np.sqrt(np.einsum("ijk,ijke,ijke->", W, (U[0] - Ue[0]) ** 2, J) / np.einsum("ijk,ijke->", W, J)))
+    rate = np.log2(errs[0] / errs[1])
+    assert rate >= 0.8 * (N + 1), (errs, rate)
+
+
+def test_lserk_coefficients_fourth_order():
+    """The 2N-storage Carpenter-Kennedy RK4-5 coefficients (timedisc_vars.f90:213-227) integrate y'=lambda y
+    with order 4, applied with the register algebra of VAXPB_OUT_VAXPB_IN (vector.f90:180-181)."""
+    A, b, c = O.lserk_coefs()
+    lam = -1.0 + 2.0j
+
+    def run(dt, T_end=1.0):
+        y = 1.0 + 0j
+        k = 0.0 + 0j
+        for _ in range(int(round(T_end / dt))):
+            for s in range(5):
+                k = k * (-A[s] if s else 0.0) + lam * y
+                y = y + b[s] * dt * k
+        return y
+
+    e = [abs(run(dt) - np.exp(lam)) for dt in (0.1, 0.05)]
+    assert np.log2(e[0] / e[1]) > 3.8
+    # the nodes c_s are the accumulated b of the previous stages
+    cum, kk = [0.0], 0.0
+    for s in range(1, 5):
+        kk = kk * (-A[s - 1] if s > 1 else 0.0) + 1.0
+        cum.append(cum[-1] + b[s - 1] * kk)
+    assert np.allclose(c[1:], cum[1:], rtol=1e-12)
+
+
+def test_calc_dt_closed_form():
+    """Cartesian box, uniform state: dt = min(CFL*alpha_N/(2N+1) * 2/lambda_c, DFL*alpha_v*1.853/(2N+1)^2 *
+    4/lambda_v) with lambda_c = sum_d (|u_d| + c) 2/h_d and lambda_v = sum_d mu/rho max(4/3,kappa/Pr) (2/h_d)^2."""
+    N, ne, L = 5, 4, 2.0
+    m = M.box_mesh((ne, ne, ne), N, lo=(0, 0, 0), hi=(L, L, L))
+    vel, rho, p, kappa, Pr, mu = (0.3, -0.2, 0.1), 1.2, 0.9, 1.4, 0.72, 0.05
+    U = T.freestream(m.Elem_xGP, rho=rho, vel=vel, p=p, kappa=kappa)
+    op = O.Operator(m, split=True, riemann=O.RIEMANN_LF, kappa=kappa, R=1.0, Pr=Pr, mu0=mu)
+    dt, err, dtc, dtv = op.calc_dt(U, 0.9, 0.8)
+    h = L / ne
+    c = np.sqrt(kappa * p / rho)
+    lc = sum((abs(v) + c) * 2 / h for v in vel)
+    lv = 3 * mu / rho * max(4 / 3, kappa / Pr) * (2 / h) ** 2
+    dtc_ref = 0.9 * 2.2027 / (2 * N + 1) * 2 / lc
+    dtv_ref = 0.8 * 0.56 * 1.853 / (2 * N + 1) ** 2 * 4 / lv
+    assert err == 0
+    assert abs(dtc - dtc_ref) < 1e-12 * dtc_ref and abs(dtv - dtv_ref) < 1e-12 * dtv_ref
+    assert dt == min(dtc, dtv)
+
+
+def test_calc_dt_err_type():
+    m = M.box_mesh((2, 2, 2), 3)
+    U = T.freestream(m.Elem_xGP).copy(order="F")
+    U[0, 0, 0, 0, 1] = -0.5
+    _dt, err, _c, _v = O.Operator(m).calc_dt(U)
+    assert err == 1
