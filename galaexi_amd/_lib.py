@@ -20,6 +20,8 @@ EXPORTS = (
     "galx_set_state", "galx_get_state", "galx_set_state_device", "galx_get_state_device", "galx_rhs", "galx_get_ut",
     "galx_get_ut_device", "galx_lserk_step", "galx_lserk_steps", "galx_calc_dt", "galx_prolong_to_face",
     "galx_surf_int", "galx_timers_enable", "galx_timer_read", "galx_timers_reset", "galx_memory_bytes",
+    "galx_rhs_vjp", "galx_rhs_vjp_device", "galx_lserk_forward_ckpt", "galx_lserk_adjoint",
+    "galx_lserk_adjoint_device",
 )
 
 
@@ -42,6 +44,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise GalxError(f"{LIB_PATH} is missing: build the HIP extension first (python -m galaexi_amd.build)")
+    # One HIP runtime per process: torch ships its own libamdhip64/libhsa-runtime64 (same sonames as ROCm's),
+    # and a second runtime instance loaded after the first cannot open the device.  Loading torch first makes
+    # libgalx bind to torch's instance (sonames match), so device pointers and streams are shared with torch.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, ip, dp, i, d = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int, C.c_double
     sig = {
@@ -67,6 +76,11 @@ def lib():
         "galx_timer_read": ([vp, C.c_char_p, dp, C.POINTER(C.c_long)], i),
         "galx_timers_reset": ([vp], i),
         "galx_memory_bytes": ([vp, C.POINTER(C.c_longlong)], i),
+        "galx_rhs_vjp": ([vp, dp, dp], i),
+        "galx_rhs_vjp_device": ([vp, vp, vp], i),
+        "galx_lserk_forward_ckpt": ([vp, d, d, i, i], i),
+        "galx_lserk_adjoint": ([vp, dp], i),
+        "galx_lserk_adjoint_device": ([vp, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -2,6 +2,7 @@
 // stage orchestration and the reference-layout unit entry points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -11,6 +12,7 @@
 #include <vector>
 
 #include "../../include/galx.h"
+#include "galx_adjoint.hpp"
 #include "galx_kernels.hpp"
 
 using namespace galx;
@@ -225,6 +227,45 @@ struct Launch {
     }
 };
 
+// discrete adjoint (split PI + LF, FP64)
+struct AdjSet {
+    bool ok = false;
+    void (*pre)(hipStream_t, int, double*, const double*, double, Mesh, double, double, double*) = nullptr;
+    void (*main)(hipStream_t, int, const double*, Mesh, const double*, double*, const double*, const double*, GasD,
+                 double, double, double*, int, int, double, double*) = nullptr;
+    void (*face)(hipStream_t, int, const double*, Mesh, const double*, GasD, double*) = nullptr;
+};
+
+template <int N>
+struct AdjLaunch {
+    using D = Dm<N>;
+    static void pre(hipStream_t s, int nE, double* kbar, const double* Ubar, double bdt, Mesh m, double LHm,
+                    double LHp, double* mutr)
+    {
+        hipLaunchKernelGGL((k_adj_pre<N>), dim3(nE), dim3(D::block), 0, s, kbar, Ubar, bdt, m, LHm, LHp, mutr);
+    }
+    static void main(hipStream_t s, int nE, const double* U, Mesh m, const double* utr, double* lam,
+                     const double* mutr, const double* mats, GasD g, double LHm, double LHp, double* Ubar, int acc,
+                     int kscale, double a, double* nghtr)
+    {
+        hipLaunchKernelGGL((k_adj_main<N>), dim3(nE), dim3(D::block), 0, s, U, m, utr, lam, mutr, mats, g, LHm, LHp,
+                           Ubar, acc, kscale, a, nghtr);
+    }
+    static void face(hipStream_t s, int nE, const double* U, Mesh m, const double* nghtr, GasD g, double* Ubar)
+    {
+        hipLaunchKernelGGL((k_adj_face<N>), dim3(nE), dim3(D::block), 0, s, U, m, nghtr, g, Ubar);
+    }
+    static AdjSet make()
+    {
+        AdjSet a;
+        a.ok = true;
+        a.pre = &pre;
+        a.main = &main;
+        a.face = &face;
+        return a;
+    }
+};
+
 #ifndef GALX_DEGREES_FP64
 #define GALX_DEGREES_FP64 X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
 #endif
@@ -247,6 +288,15 @@ static KernelSet select_kernels(int N, int prec, int form)
 #undef X
     }
     return KernelSet{};
+}
+
+static AdjSet select_adjoint(int N)
+{
+#define X(NN) \
+    if (N == NN) return AdjLaunch<NN>::make();
+    GALX_DEGREES_FP64
+#undef X
+    return AdjSet{};
 }
 
 // ----------------------------------------------------------------------------------------------------------
@@ -277,6 +327,15 @@ struct galx_ctx {
     int* dErr = nullptr;
     int cur = 0;
     bool meshSet = false, stateSet = false, utValid = false;
+    // discrete adjoint: cotangents, face traces and the recorded forward trajectory (element planes)
+    AdjSet as;
+    double *dKbar = nullptr, *dUbar = nullptr, *dMutr = nullptr, *dNghtr = nullptr;
+    std::vector<double*> ckpt;   // U at steps 0, C, 2C, ... of the recorded trajectory
+    std::vector<double*> stages; // stage inputs of one segment of C steps (5 per step)
+    double *dUfinal = nullptr;
+    double trajT = 0.0, trajDt = 0.0;
+    int trajSteps = 0, trajEvery = 0;
+    bool trajValid = false;
     long long bytes = 0;
     bool timing = false;
     std::map<std::string, Timer> timers;
@@ -383,6 +442,8 @@ int galx_create(const galx_config* cfg, galx_ctx** out)
     c->n2 = c->n * c->n;
     c->n3 = c->n2 * c->n;
     c->ks = ks;
+    if (cfg->precision == GALX_PREC_FP64 && cfg->form == GALX_FORM_SPLIT_PI && cfg->riemann == GALX_RIEMANN_LF)
+        c->as = select_adjoint(cfg->N);
     c->tsz = cfg->precision == GALX_PREC_FP64 ? 8 : 4;
     c->basis = make_basis(cfg->N);
     // THERMAL_CONDUCTIVITY_EOS (src/equations/navierstokes/idealgas/eos.h:110)
@@ -426,6 +487,10 @@ int galx_destroy(galx_ctx* c)
     void* ps[] = {c->dU, c->dK, c->dUt, c->dScratch, c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dUtr[0], c->dUtr[1],
                   c->dGtr, c->dMats, c->dDtc, c->dDtv, c->dDtOut, c->dErr};
     for (void* p : ps) dfree(p);
+    void* pa[] = {c->dKbar, c->dUbar, c->dMutr, c->dNghtr, c->dUfinal};
+    for (void* p : pa) dfree(p);
+    for (double* p : c->ckpt) dfree(p);
+    for (double* p : c->stages) dfree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GALX_OK;
@@ -777,6 +842,209 @@ int galx_memory_bytes(galx_ctx* c, long long* bytes)
 {
     if (!c || !bytes) return fail(GALX_E_ARG, "null argument");
     *bytes = c->bytes;
+    return GALX_OK;
+}
+
+// ----------------------------------------------------------------------------------------------------------
+// discrete adjoint
+// ----------------------------------------------------------------------------------------------------------
+static int adj_check(galx_ctx* c)
+{
+    if (!c) return fail(GALX_E_ARG, "null context");
+    if (!c->as.ok)
+        return fail(GALX_E_ARG, "the discrete adjoint is implemented for form SPLIT_PI, riemann LF, precision FP64");
+    if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    long nDOF = (long)c->nElems * c->n3, nfn = (long)c->nSides * c->n2;
+    int rc;
+    if ((rc = dalloc(c, &c->dKbar, 5 * nDOF * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dUbar, 5 * nDOF * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dMutr, 2 * 5 * nfn * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dNghtr, 2 * 4 * nfn * sizeof(double)))) return rc;
+    return GALX_OK;
+}
+
+// Ubar (+)= (dR/dU(U))^T lam for U in element planes with its FP64 trace records utr; lam is consumed:
+// lam <- lam + bdt*Ubar first when Ubar accumulates (LSERK stage), and lam <- a*lam at the end when kscale
+static void adj_apply(galx_ctx* c, const double* U, const double* utr, double* lam, double* Ubar, int accumulate,
+                      double bdt, int kscale, double a)
+{
+    TimerScope ts(c, "adjoint");
+    const Basis& B = c->basis;
+    double LHm = 1.0 / B.w[0], LHp = 1.0 / B.w[c->N];
+    Mesh m = mesh_of(c);
+    c->as.pre(c->stream, c->nElems, lam, accumulate ? Ubar : nullptr, bdt, m, LHm, LHp, c->dMutr);
+    c->as.main(c->stream, c->nElems, U, m, utr, lam, c->dMutr, (const double*)c->dMats, gas_of(c), LHm, LHp, Ubar,
+               accumulate, kscale, a, c->dNghtr);
+    c->as.face(c->stream, c->nElems, U, m, c->dNghtr, gas_of(c), Ubar);
+}
+
+int galx_rhs_vjp_device(galx_ctx* c, const double* dLam, double* dUbarOut)
+{
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!dLam || !dUbarOut) return fail(GALX_E_ARG, "null argument");
+    long nDOF = (long)c->nElems * c->n3;
+    unsigned g = (unsigned)((nDOF + 255) / 256);
+    k_ref_to_planes<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, dLam, c->dKbar);
+    HIPCHK(hipGetLastError());
+    adj_apply(c, c->dU, (const double*)c->dUtr[c->cur], c->dKbar, c->dUbar, 0, 0.0, 0, 0.0);
+    HIPCHK(hipGetLastError());
+    k_planes_to_ref<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUbar, dUbarOut);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+int galx_rhs_vjp(galx_ctx* c, const double* lam, double* UbarOut)
+{
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!lam || !UbarOut) return fail(GALX_E_ARG, "null argument");
+    long nDOF = (long)c->nElems * c->n3;
+    if ((rc = dalloc(c, &c->dScratch, 5 * nDOF * sizeof(double)))) return rc;
+    unsigned g = (unsigned)((nDOF + 255) / 256);
+    HIPCHK(hipMemcpyAsync(c->dScratch, lam, 5 * nDOF * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    k_ref_to_planes<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dScratch, c->dKbar);
+    HIPCHK(hipGetLastError());
+    adj_apply(c, c->dU, (const double*)c->dUtr[c->cur], c->dKbar, c->dUbar, 0, 0.0, 0, 0.0);
+    HIPCHK(hipGetLastError());
+    k_planes_to_ref<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUbar, c->dScratch);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(UbarOut, c->dScratch, 5 * nDOF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+int galx_lserk_forward_ckpt(galx_ctx* c, double t, double dt, int nSteps, int ckptEvery)
+{
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (nSteps < 1 || ckptEvery < 1 || !(dt == dt)) return fail(GALX_E_ARG, "bad step arguments");
+    size_t ub = 5L * c->nElems * c->n3 * sizeof(double);
+    int nSeg = (nSteps + ckptEvery - 1) / ckptEvery;
+    int nStage = 5 * std::min(ckptEvery, nSteps);
+    // storage: checkpoints, the stage inputs of one segment, the final state
+    while ((int)c->ckpt.size() > nSeg) {
+        dfree(c->ckpt.back());
+        c->bytes -= (long long)ub;
+        c->ckpt.pop_back();
+    }
+    while ((int)c->stages.size() > nStage) {
+        dfree(c->stages.back());
+        c->bytes -= (long long)ub;
+        c->stages.pop_back();
+    }
+    while ((int)c->ckpt.size() < nSeg) {
+        double* p = nullptr;
+        if ((rc = dalloc(c, &p, ub))) return fail(rc, "checkpoint storage (%d x %zu bytes): %s", nSeg, ub, g_err.c_str());
+        c->ckpt.push_back(p);
+    }
+    while ((int)c->stages.size() < nStage) {
+        double* p = nullptr;
+        if ((rc = dalloc(c, &p, ub)))
+            return fail(rc, "stage storage (%d x %zu bytes; lower ckptEvery): %s", nStage, ub, g_err.c_str());
+        c->stages.push_back(p);
+    }
+    if ((rc = dalloc(c, &c->dUfinal, ub))) return rc;
+    c->trajValid = false;
+    for (int seg = 0; seg < nSeg; seg++) {
+        HIPCHK(hipMemcpyAsync(c->ckpt[seg], c->dU, ub, hipMemcpyDeviceToDevice, c->stream));
+        int n = std::min(ckptEvery, nSteps - seg * ckptEvery);
+        if ((rc = lserk_enqueue(c, dt, n))) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(c->dUfinal, c->dU, ub, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->trajT = t;
+    c->trajDt = dt;
+    c->trajSteps = nSteps;
+    c->trajEvery = ckptEvery;
+    c->trajValid = true;
+    return GALX_OK;
+}
+
+// reverse sweep over the recorded trajectory; dUbar (element planes) holds dJ/dU(final) on entry and
+// dJ/dU(initial) on exit.  The LSERK stage (src/timedisc/timestep.f90:85-119)
+//   k' = a k + R(U),  U' = U + b dt k'
+// has the adjoint  kbar_tot = kbar' + b dt Ubar',  Ubar = Ubar' + R'(U)^T kbar_tot,  kbar = a kbar_tot.
+static int adj_reverse(galx_ctx* c)
+{
+    size_t ub = 5L * c->nElems * c->n3 * sizeof(double);
+    const int C = c->trajEvery, nSteps = c->trajSteps;
+    const int nSeg = (nSteps + C - 1) / C;
+    Mesh m = mesh_of(c);
+    GasD gd = gas_of(c);
+    double* utrA = (double*)c->dUtr[0];
+    HIPCHK(hipMemsetAsync(c->dKbar, 0, ub, c->stream));
+    for (int seg = nSeg - 1; seg >= 0; seg--) {
+        const int s0 = seg * C, n = std::min(C, nSteps - s0);
+        // recompute the segment, keeping every stage input
+        HIPCHK(hipMemcpyAsync(c->dU, c->ckpt[seg], ub, hipMemcpyDeviceToDevice, c->stream));
+        c->cur = 0;
+        c->ks.trace(c->stream, c->nElems, c->dU, m, c->dUtr[0], gd);
+        for (int s = 0; s < n; s++)
+            for (int stage = 0; stage < 5; stage++) {
+                HIPCHK(hipMemcpyAsync(c->stages[5 * s + stage], c->dU, ub, hipMemcpyDeviceToDevice, c->stream));
+                launch_lift(c);
+                double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
+                launch_rhs(c, Stage{mRKA, RKB[stage] * c->trajDt, 1});
+                c->cur ^= 1;
+            }
+        HIPCHK(hipGetLastError());
+        // reverse
+        for (int s = n - 1; s >= 0; s--)
+            for (int stage = 4; stage >= 0; stage--) {
+                const double* Us = c->stages[5 * s + stage];
+                c->ks.trace(c->stream, c->nElems, Us, m, utrA, gd);
+                double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
+                adj_apply(c, Us, utrA, c->dKbar, c->dUbar, 1, RKB[stage] * c->trajDt, 1, mRKA);
+            }
+        HIPCHK(hipGetLastError());
+    }
+    // back to the final state of the trajectory
+    HIPCHK(hipMemcpyAsync(c->dU, c->dUfinal, ub, hipMemcpyDeviceToDevice, c->stream));
+    c->cur = 0;
+    c->ks.trace(c->stream, c->nElems, c->dU, m, c->dUtr[0], gd);
+    HIPCHK(hipMemsetAsync(c->dK, 0, ub, c->stream));
+    HIPCHK(hipGetLastError());
+    c->utValid = false;
+    return GALX_OK;
+}
+
+int galx_lserk_adjoint_device(galx_ctx* c, double* dUbar)
+{
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!dUbar) return fail(GALX_E_ARG, "null argument");
+    if (!c->trajValid) return fail(GALX_E_STATE, "record a trajectory with galx_lserk_forward_ckpt first");
+    long nDOF = (long)c->nElems * c->n3;
+    unsigned g = (unsigned)((nDOF + 255) / 256);
+    k_ref_to_planes<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, dUbar, c->dUbar);
+    HIPCHK(hipGetLastError());
+    if ((rc = adj_reverse(c))) return rc;
+    k_planes_to_ref<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUbar, dUbar);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+int galx_lserk_adjoint(galx_ctx* c, double* Ubar)
+{
+    int rc = adj_check(c);
+    if (rc) return rc;
+    if (!Ubar) return fail(GALX_E_ARG, "null argument");
+    if (!c->trajValid) return fail(GALX_E_STATE, "record a trajectory with galx_lserk_forward_ckpt first");
+    long nDOF = (long)c->nElems * c->n3;
+    if ((rc = dalloc(c, &c->dScratch, 5 * nDOF * sizeof(double)))) return rc;
+    unsigned g = (unsigned)((nDOF + 255) / 256);
+    HIPCHK(hipMemcpyAsync(c->dScratch, Ubar, 5 * nDOF * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    k_ref_to_planes<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dScratch, c->dUbar);
+    HIPCHK(hipGetLastError());
+    if ((rc = adj_reverse(c))) return rc;
+    k_planes_to_ref<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUbar, c->dScratch);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(Ubar, c->dScratch, 5 * nDOF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return GALX_OK;
 }
 

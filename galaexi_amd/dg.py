@@ -121,6 +121,34 @@ class DGOperator:
         check(lib().galx_calc_dt(self._h, C.byref(dt), C.byref(err)), "galx_calc_dt")
         return dt.value, err.value
 
+    # discrete adjoint (split PI + LF, FP64)
+    def rhs_vjp(self, lam):
+        """Ubar = (dR/dU)^T lam at the current state (host arrays in the U layout)."""
+        lam = _F(lam)
+        assert lam.shape == self._shape()
+        Ubar = np.zeros(self._shape(), order="F")
+        check(lib().galx_rhs_vjp(self._h, _dptr(lam), _dptr(Ubar)), "galx_rhs_vjp")
+        return Ubar
+
+    def rhs_vjp_device(self, lam_ptr, ubar_ptr):
+        check(lib().galx_rhs_vjp_device(self._h, C.c_void_p(lam_ptr), C.c_void_p(ubar_ptr)), "galx_rhs_vjp_device")
+
+    def forward_ckpt(self, t, dt, nSteps, ckptEvery):
+        """TimeStepByLSERKW2 over nSteps with the trajectory recorded for lserk_adjoint."""
+        check(lib().galx_lserk_forward_ckpt(self._h, float(t), float(dt), int(nSteps), int(ckptEvery)),
+              "galx_lserk_forward_ckpt")
+        return t + nSteps * dt
+
+    def lserk_adjoint(self, Ubar_final):
+        """dJ/dU(initial) from dJ/dU(final) over the recorded trajectory."""
+        Ubar = _F(Ubar_final).copy(order="F")
+        assert Ubar.shape == self._shape()
+        check(lib().galx_lserk_adjoint(self._h, _dptr(Ubar)), "galx_lserk_adjoint")
+        return Ubar
+
+    def lserk_adjoint_device(self, ubar_ptr):
+        check(lib().galx_lserk_adjoint_device(self._h, C.c_void_p(ubar_ptr)), "galx_lserk_adjoint_device")
+
     def timers(self, on=True):
         check(lib().galx_timers_enable(self._h, int(on)))
 

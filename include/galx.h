@@ -111,9 +111,27 @@ int galx_surf_int(int device, int nVar, int N, int nElems, int nSides, const int
                   const double *Flux_master, const double *Flux_slave, double L_HatMinus0, double L_HatPlusN,
                   double *Ut);
 
+/* Discrete adjoint (form SPLIT_PI, riemann LF, precision FP64; GALX_E_ARG otherwise).  The reference has no
+ * adjoint; its nearest interface is the matrix-free Jacobian-vector product of the implicit module,
+ * MatrixVector (src/implicit/implicit.f90:480-528), which forms (R(U+eps v) - R(U))/eps by finite differences.
+ * These entry points give the exact transpose of that Jacobian: Ubar = (dR/dU)^T lam, to rounding.
+ * galx_rhs_vjp: at the current state, lam and Ubar in the U layout (host / device pointers). */
+int galx_rhs_vjp(galx_ctx *ctx, const double *lam, double *Ubar);
+int galx_rhs_vjp_device(galx_ctx *ctx, const double *dLam, double *dUbar);
+/* galx_lserk_forward_ckpt: TimeStepByLSERKW2 for nSteps steps of fixed dt from the current state, recording
+ * the trajectory for the reverse sweep: the state every ckptEvery steps is kept on the device, together with
+ * room for the 5*min(ckptEvery,nSteps) stage inputs of one segment (device memory ~ (nSteps/ckptEvery +
+ * 5*ckptEvery + 1) copies of U).  The context ends at the final state. */
+int galx_lserk_forward_ckpt(galx_ctx *ctx, double t, double dt, int nSteps, int ckptEvery);
+/* galx_lserk_adjoint: reverse sweep over the recorded trajectory (segments recomputed from their
+ * checkpoints).  Ubar holds dJ/dU(final) on entry and dJ/dU(initial) on exit (U layout, host / device).
+ * The context is left at the final state of the trajectory, which stays recorded. */
+int galx_lserk_adjoint(galx_ctx *ctx, double *Ubar);
+int galx_lserk_adjoint_device(galx_ctx *ctx, double *dUbar);
+
 /* Kernel timers: when enabled, every kernel launch of the context is bracketed with HIP events on the
  * context stream; galx_timer_read returns the accumulated milliseconds and launch count of a kernel family
- * ("trace", "lift", "rhs", "dt"). */
+ * ("trace", "lift", "rhs", "dt", "adjoint"). */
 int galx_timers_enable(galx_ctx *ctx, int on);
 int galx_timer_read(galx_ctx *ctx, const char *name, double *ms, long *count);
 int galx_timers_reset(galx_ctx *ctx);
