@@ -1,13 +1,15 @@
 """Benchmark of the DGSEM RHS + LSERK hot path on one MI355X.
 
-Default workload (the headline configuration): Taylor-Green vortex Re=1600, Ma=0.1 on [0,2pi]^3, 32^3
-elements, N=7 Gauss-Lobatto (16.8M DOF), split-form Pirozzoli volume flux, Lax-Friedrichs surface flux, BR1
-lifting, Carpenter 5-stage LSERK, FP64, synthetic initial state with seeded 1e-3 velocity noise, dt from
-CALCTIMESTEP at t=0 held fixed.  One "step" = one LSERK step (5 RHS stages) over the whole mesh.
-value = nDOF * 5 * steps / wall seconds (DOF-updates/s), inputs resident in HBM before the timed region.
+Headline workload: Taylor-Green vortex Re=1600, Ma=0.1 on [0,2pi]^3, 32^3 elements, N=7 Gauss-Lobatto (16.8M
+DOF), split-form Pirozzoli volume flux, Lax-Friedrichs surface flux, BR1 lifting, Carpenter 5-stage LSERK,
+FP64, synthetic initial state with seeded 1e-3 velocity noise, dt from CALCTIMESTEP at t=0 held fixed.  One
+"step" = one LSERK step (5 RHS stages) over the whole mesh.  value = nDOF * 5 * steps / wall seconds
+(DOF-updates/s), inputs resident in HBM before the timed region.
 
-Also reported (extra keys): PID, the same configuration at 48^3 in FP64 and in the mixed form with their
-ratio, per-kernel times, the roofline of the dominant kernel and the oracle's CPU throughput.
+Also reported (extra keys): PID, the same configuration at 48^3 in FP64 and in the mixed form with their ratio
+and per-variable error, the discrete adjoint on the curved 32^3 mesh (gradient of the kinetic energy after 400
+steps, checkpoints every 20: adjoint/forward cost ratio and a finite-difference check), per-kernel times, the
+HBM roofline of the dominant kernel and the oracle's CPU throughput.
 """
 from __future__ import annotations
 
@@ -15,7 +17,6 @@ import argparse
 import json
 import os
 import platform
-import subprocess
 import sys
 import time
 
@@ -27,22 +28,41 @@ sys.path.insert(0, ROOT)
 METRIC = "PID & DOF-updates/s, FP64 vs mixed at stated L2; adjoint/forward cost ratio"
 HBM_PEAK_GBS = 8000.0
 NSTAGES = 5
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rhs_k_lift_tgv_n7_32cubed_fp64_traffic.json")
 
-# algorithmic HBM bytes per DOF-stage for the two passes of one stage (3-pass BR1 model with the trace pass
-# fused into the update pass): state b_s=8, geometry b_g=8, traces b_t (8 FP64 / 4 mixed), phi = 6/(N+1)
+
 def alg_bytes(N, b_t):
+    """Algorithmic HBM bytes per DOF for one stage's two kernels (b_t: bytes per trace value, 8 in FP64, 4 in the
+    mixed form; phi = 6 n^2 / n^3 face nodes per volume node):
+      k_lift: read U (5x8), metrics+sJ (10x8), face normal+SurfElem (phi 4x8), neighbour U traces (phi 5 b_t);
+              write own gradient traces (phi 12 b_t)
+      k_rhs : read U (5x8), metrics+sJ (10x8), face geometry (phi 10x8), neighbour U and gradient traces
+              (phi 17 b_t), LSERK register (5x8); write U (5x8), register (5x8), own next U traces (phi 5 b_t)"""
     phi = 6.0 / (N + 1)
-    b_s = b_g = 8.0
-    lift = 5 * b_s + 10 * b_g + phi * 5 * b_t + phi * 4 * b_g + phi * 15 * b_t
-    rhs = 5 * b_s + 10 * b_g + phi * 10 * b_g + phi * 5 * b_t + phi * 15 * b_t + 3 * 5 * 8 + phi * 5 * b_t
+    lift = 5 * 8 + 10 * 8 + phi * (4 * 8 + 5 * b_t + 12 * b_t)
+    rhs = 5 * 8 + 10 * 8 + phi * 10 * 8 + phi * 17 * b_t + 5 * 8 + 5 * 8 + 5 * 8 + phi * 5 * b_t
     return lift, rhs
 
 
-def tgv_case(ne, N=7, noise=1e-3):
+def load_traffic(kernel, ne, N):
+    """HBM bytes per launch of a kernel of the headline workload from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a plain copy kernel's readings), or None for other workloads."""
+    if ne != 32 or N != 7 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    d = json.load(open(TRAFFIC_FILE))
+    k = d["kernels"].get(kernel)
+    if k is None:
+        return None
+    cal = d["copy_calibration"]
+    return 1024.0 * (k["FETCH_SIZE_KiB"] * cal["fetch_scale"] + k["WRITE_SIZE_KiB"] * cal["write_scale"])
+
+
+def tgv_case(ne, N=7, noise=1e-3, curved=False):
     from galaexi_amd import mesh as M
     from galaexi_amd import testcases as T
 
-    m = M.box_mesh((ne, ne, ne), N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3)
+    kw = dict(deform_amp=0.05, deform_phase=(0.31, 0.17, 0.83)) if curved else {}
+    m = M.box_mesh((ne, ne, ne), N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3, **kw)
     U = T.taylor_green(m.Elem_xGP, Ma=0.1, noise=noise, seed=1600)
     return m, U
 
@@ -75,6 +95,89 @@ def run_gpu(m, U, precision, steps, warmup, dt=None):
     return dict(wall=t1 - t0, kernels=ker, dt=dt, U=Uend, mem=mem)
 
 
+def mass_weights(m):
+    from galaexi_amd import basis
+
+    _x, w, _wb = basis.nodes_and_weights(m.N, basis.GAUSS_LOBATTO)
+    W = w[:, None, None, None] * w[None, :, None, None] * w[None, None, :, None]
+    return W / m.sJ
+
+
+def kinetic_energy(W, U):
+    return float(np.sum(W * 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / U[0]))
+
+
+def kinetic_energy_grad(W, U):
+    g = np.zeros_like(U, order="F")
+    g[0] = -0.5 * W * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / U[0] ** 2
+    for d in (1, 2, 3):
+        g[d] = W * U[d] / U[0]
+    return g
+
+
+def run_adjoint(ne, steps, every, fd_eps):
+    """J = int 1/2 rho |u|^2 dV after `steps` LSERK steps of the curved TGV case (FP64); dJ/dU0 by the
+    checkpointed discrete adjoint.  Cost ratio = wall of the reverse sweep (segment recompute + stage VJPs)
+    over the wall of the plain forward integration; a central finite difference along one seeded direction
+    checks the gradient."""
+    from galaexi_amd import dg
+
+    N = 7
+    m, U0 = tgv_case(ne, N, curved=True)
+    W = mass_weights(m)
+    op = dg.DGOperator(N, form="split", riemann="lf", precision="fp64", **TGV_GAS)
+    op.set_mesh(m)
+    op.set_state(U0)
+    dt, err = op.CalcTimeStep()
+    assert err == 0, err
+    op.TimeStepByLSERKW2(0.0, dt, 2)  # warm-up
+    op.set_state(U0)
+    t0 = time.perf_counter()
+    op.TimeStepByLSERKW2(0.0, dt, steps)
+    t_fwd = time.perf_counter() - t0
+    op.set_state(U0)
+    t0 = time.perf_counter()
+    op.forward_ckpt(0.0, dt, steps, every)
+    t_rec = time.perf_counter() - t0
+    UN = op.get_state()
+    J = kinetic_energy(W, UN)
+    op.timers(True)
+    op.timers_reset()
+    t0 = time.perf_counter()
+    g0 = op.lserk_adjoint(kinetic_energy_grad(W, UN))
+    t_adj = time.perf_counter() - t0
+    ker = {k: op.timer(k) for k in ("trace", "lift", "rhs", "adjoint")}
+    op.timers(False)
+    mem = op.memory_bytes()
+    rng = np.random.default_rng(400)
+    s = np.abs(U0).reshape(5, -1).mean(axis=1).reshape(5, 1, 1, 1, 1)
+    v = np.asfortranarray(0.1 * s * rng.standard_normal(U0.shape))
+    exact = float(np.sum(g0 * v))
+    fd = {}
+    for eps in fd_eps:
+        Js = []
+        for sg in (1.0, -1.0):
+            op.set_state(np.asfortranarray(U0 + sg * eps * v))
+            op.TimeStepByLSERKW2(0.0, dt, steps)
+            Js.append(kinetic_energy(W, op.get_state()))
+        d = (Js[0] - Js[1]) / (2 * eps)
+        fd[f"{eps:g}"] = abs(d - exact) / abs(exact)
+    op.close()
+    return {
+        "workload": f"TGV Re1600 Ma0.1 [0,2pi]^3 {ne}^3 elems N={N}, curved (0.05 sine mapping), split-PI + LF, "
+                    f"BR1, FP64; J = int 1/2 rho|u|^2 dV after {steps} LSERK steps, checkpoints every {every}",
+        "nDOF": m.nDOF, "dt": dt, "J": J,
+        "forward_s": t_fwd, "forward_with_checkpoints_s": t_rec, "adjoint_sweep_s": t_adj,
+        "adjoint_over_forward": t_adj / t_fwd,
+        "gradient_total_over_forward": (t_rec + t_adj) / t_fwd,
+        "adjoint_kernels_ms_avg": {k: (v_[0] / v_[1] if v_[1] else 0.0) for k, v_ in ker.items()},
+        "adjoint_kernels_ms_total": {k: v_[0] for k, v_ in ker.items()},
+        "device_memory_GB": mem / 1e9,
+        "fd_check": {"direction": "seeded N(0,1) x 0.1 mean|U_v| per variable", "dJ_dot_v": exact,
+                     "central_fd_rel_err_by_eps": fd, "best": min(fd.values())},
+    }
+
+
 def cpu_baseline(N=7, ne=16):
     """The FP64 oracle (C/OpenMP restatement of the reference operator) on all host cores, one LSERK step of
     the same TGV configuration on a 16^3 mesh (same per-DOF work as the headline mesh)."""
@@ -105,7 +208,10 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ne", type=int, default=32, help="elements per direction of the headline workload")
     ap.add_argument("--compare-ne", type=int, default=48, help="mesh for the FP64-vs-mixed comparison (0: skip)")
-    ap.add_argument("--compare-steps", type=int, default=10)
+    ap.add_argument("--compare-steps", type=int, default=1000)
+    ap.add_argument("--adjoint-ne", type=int, default=32, help="mesh of the adjoint measurement (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=400)
+    ap.add_argument("--adjoint-ckpt", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -143,11 +249,14 @@ def main(argv=None):
                    "dt": r["dt"]},
         "pid_s": 1.0 / value,
         "kernels_ms_avg": {"lift": 1e3 * lift_avg_s, "rhs": 1e3 * rhs_avg_s},
-        "roofline": {"bound": "hbm", "kernel": "k_rhs<7,double>", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_dof": rhs_b},
+        "roofline": {"bound": "hbm", "kernel": "k_rhs<7,double,double,split,LF>", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic("k_rhs", a.ne, N), "alg_bytes_per_dof": rhs_b,
+                     "alg_bytes_per_launch": rhs_b * nDOF},
         "stage_roofline": {"alg_bytes_per_dof_stage": lift_b + rhs_b,
-                           "achieved_GBs": (lift_b + rhs_b) * nDOF / (lift_avg_s + rhs_avg_s) / 1e9},
+                           "achieved_GBs": (lift_b + rhs_b) * nDOF / (lift_avg_s + rhs_avg_s) / 1e9,
+                           "k_lift_achieved_GBs": lift_b * nDOF / lift_avg_s / 1e9,
+                           "k_lift_traffic": load_traffic("k_lift", a.ne, N)},
         "device_memory_GB": r["mem"] / 1e9,
     }
     del m, U, r
@@ -165,11 +274,16 @@ def main(argv=None):
                           "fp64_dof_updates_per_s": v64, "mixed_dof_updates_per_s": v32,
                           "mixed_over_fp64": v32 / v64,
                           "mixed_rel_l2_per_var_after_steps": l2,
+                          "mixed_alg_bytes_per_dof_stage": sum(alg_bytes(N, 4)),
                           "fp64_kernels_ms_avg": {k: r64["kernels"][k][0] / r64["kernels"][k][1]
                                                   for k in ("lift", "rhs")},
                           "mixed_kernels_ms_avg": {k: r32["kernels"][k][0] / r32["kernels"][k][1]
                                                    for k in ("lift", "rhs")}}
         del m2, U2, r64, r32
+    if a.adjoint_ne:
+        print(f"# adjoint: curved TGV N={N} {a.adjoint_ne}^3, {a.adjoint_steps} steps, checkpoints every "
+              f"{a.adjoint_ckpt}", file=sys.stderr, flush=True)
+        out["adjoint"] = run_adjoint(a.adjoint_ne, a.adjoint_steps, a.adjoint_ckpt, (1e-5, 1e-6))
     if not a.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

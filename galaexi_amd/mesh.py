@@ -17,6 +17,8 @@ All arrays are numpy arrays in Fortran order indexed exactly like the reference 
 from __future__ import annotations
 
 import itertools
+import os
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -131,9 +133,23 @@ class Mesh:
         return self.nElems * (self.N + 1) ** 3
 
 
+def _along_i(V, X):
+    """Apply the 1D operator V(a,i) along the third-last axis of X(..., i, j, k)."""
+    ni, nj, nk = X.shape[-3:]
+    return np.matmul(V, X.reshape(X.shape[:-3] + (ni, nj * nk))).reshape(X.shape[:-3] + (V.shape[0], nj, nk))
+
+
+def _along_j(V, X):
+    return np.matmul(V, X)
+
+
+def _along_k(V, X):
+    return np.matmul(X, V.T)
+
+
 def _tensor3(V1, V2, V3, X):
-    """Apply 1D operators along axes 1,2,3 of X(c, i, j, k, e)."""
-    return np.einsum("ai,bj,ck,xijke->xabce", V1, V2, V3, X, optimize=True)
+    """Apply 1D operators along the last three axes of X(..., i, j, k)."""
+    return _along_i(V1, _along_j(V2, _along_k(V3, X)))
 
 
 def _surf_metrics_from_ja(normal_dir, tang_dir, normal_sign, Ja):
@@ -149,13 +165,14 @@ def _surf_metrics_from_ja(normal_dir, tang_dir, normal_sign, Ja):
 
 
 def box_mesh(nelems, N, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), deform_amp=0.0, deform_phase=(0.0, 0.0, 0.0),
-             NGeo=None, rotations=None, seed=None, chunk=2048):
+             NGeo=None, rotations=None, seed=None, chunk=256, threads=None):
     """Periodic box of nelems=(nx,ny,nz) hexahedra with polynomial degree N.
 
     deform_amp: node displacement x_d += (hi-lo)_d/2 * amp * prod_e sin(pi*(xhat_e + phase_e)), xhat in [-1,1],
         applied at the equidistant NGeo geometry nodes (2-periodic in xhat, so periodicity is kept).
     NGeo: geometry degree (default 1 for affine boxes, N for deformed ones).
     rotations: None (all elements in the global frame), 'random' (seeded), or an int array of rotation indices.
+    chunk, threads: elements per geometry batch and worker threads (default: all cores, at most 16).
     """
     nx, ny, nz = (int(v) for v in nelems)
     nE = nx * ny * nz
@@ -237,63 +254,49 @@ def box_mesh(nelems, N, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), deform_amp=0.0, 
     uniform = deform_amp == 0.0 and np.all(rot == 0)
 
     def geometry(elem_ids, centre_override=None):
+        """Metrics of a chunk of elements, element axis first: Ja(e, dir, comp, i, j, k), sJ(e, i, j, k),
+        JaCL(e, dir, comp, i, j, k) on the CL nodes of degree N, xGP(e, comp, i, j, k)."""
         ne = len(elem_ids)
-        ng = NGeo + 1
         xi = np.stack(np.meshgrid(xEQ, xEQ, xEQ, indexing="ij"), axis=0)  # (3, i, j, k)
-        R = np.stack(_ROTS)[rot[elem_ids]]  # (ne,3,3)
-        loc = np.einsum("eab,bijk->aijke", R, xi)
+        R = np.stack(_ROTS)[rot[elem_ids]].astype(np.float64)  # (ne,3,3)
+        loc = np.einsum("eab,bijk->eaijk", R, xi)
         if centre_override is not None:
-            centre = np.broadcast_to(centre_override, (ne, 3)).T
+            centre = np.broadcast_to(np.asarray(centre_override, dtype=np.float64), (ne, 3))
         else:
-            centre = (lo[:, None] + (cells[elem_ids].T + 0.5) * h[:, None])
-        X = centre[:, None, None, None, :] + 0.5 * h[:, None, None, None, None] * loc
+            centre = lo[None, :] + (cells[elem_ids] + 0.5) * h[None, :]
+        X = centre[:, :, None, None, None] + 0.5 * h[None, :, None, None, None] * loc  # (e, 3, i, j, k)
         if deform_amp != 0.0:
-            xh = 2.0 * (X - lo[:, None, None, None, None]) / (hi - lo)[:, None, None, None, None] - 1.0
-            ph = np.asarray(deform_phase, dtype=np.float64)
-            disp = deform_amp * np.prod(np.sin(np.pi * (xh + ph[:, None, None, None, None])), axis=0)
-            X = X + 0.5 * (hi - lo)[:, None, None, None, None] * disp[None]
+            span = (hi - lo)[None, :, None, None, None]
+            xh = 2.0 * (X - lo[None, :, None, None, None]) / span - 1.0
+            ph = np.asarray(deform_phase, dtype=np.float64)[None, :, None, None, None]
+            disp = deform_amp * np.prod(np.sin(np.pi * (xh + ph)), axis=1)
+            X = X + 0.5 * span * disp[:, None]
         XCLg = _tensor3(V_EQ_CL, V_EQ_CL, V_EQ_CL, X)
         XCLn = _tensor3(V_CLg_CLn, V_CLg_CLn, V_CLg_CLn, XCLg)
-        I_g = np.eye(ng)
-        dXg = np.stack([_tensor3(DCLg, I_g, I_g, XCLg), _tensor3(I_g, DCLg, I_g, XCLg),
-                        _tensor3(I_g, I_g, DCLg, XCLg)], axis=0)  # (dir, comp, i,j,k,e)
-        dXref = np.stack([_tensor3(V_CLg_Ref, V_CLg_Ref, V_CLg_Ref, dXg[d]) for d in range(3)], axis=0)
-        a = dXref
-        det = (a[0, 0] * (a[1, 1] * a[2, 2] - a[2, 1] * a[1, 2]) +
-               a[1, 0] * (a[2, 1] * a[0, 2] - a[0, 1] * a[2, 2]) +
-               a[2, 0] * (a[0, 1] * a[1, 2] - a[1, 1] * a[0, 2]))
-        detN = _tensor3(V_Ref_N, V_Ref_N, V_Ref_N, det[None])[0]
-        sJ = 1.0 / detN
+        dXg = np.stack([_along_i(DCLg, XCLg), _along_j(DCLg, XCLg), _along_k(DCLg, XCLg)], axis=1)  # (e,dir,comp,..)
+        a = _tensor3(V_CLg_Ref, V_CLg_Ref, V_CLg_Ref, dXg)
+        det = (a[:, 0, 0] * (a[:, 1, 1] * a[:, 2, 2] - a[:, 2, 1] * a[:, 1, 2]) +
+               a[:, 1, 0] * (a[:, 2, 1] * a[:, 0, 2] - a[:, 0, 1] * a[:, 2, 2]) +
+               a[:, 2, 0] * (a[:, 0, 1] * a[:, 1, 2] - a[:, 1, 1] * a[:, 0, 2]))
+        detN = _tensor3(V_Ref_N, V_Ref_N, V_Ref_N, det)
+        sJe = 1.0 / detN
         if N >= NGeo:
-            dXn = np.stack([_tensor3(V_CLg_CLn, V_CLg_CLn, V_CLg_CLn, dXg[d]) for d in range(3)], axis=0)
+            dXn = _tensor3(V_CLg_CLn, V_CLg_CLn, V_CLg_CLn, dXg)
         else:
-            I_n = np.eye(n)
-            dXn = np.stack([_tensor3(DCLn, I_n, I_n, XCLn), _tensor3(I_n, DCLn, I_n, XCLn),
-                            _tensor3(I_n, I_n, DCLn, XCLn)], axis=0)
-        # curl form (metrics.f90:450-484): R(dir, comp)
-        Xc = XCLn
+            dXn = np.stack([_along_i(DCLn, XCLn), _along_j(DCLn, XCLn), _along_k(DCLn, XCLn)], axis=1)
+        # curl form (metrics.f90:450-484): Rc(e, dir, comp)
+        Xc = XCLn[:, None]
         Rc = np.empty_like(dXn)
-        Rc[:, 0] = 0.5 * (Xc[2][None] * dXn[:, 1] - Xc[1][None] * dXn[:, 2])
-        Rc[:, 1] = 0.5 * (Xc[0][None] * dXn[:, 2] - Xc[2][None] * dXn[:, 0])
-        Rc[:, 2] = 0.5 * (Xc[1][None] * dXn[:, 0] - Xc[0][None] * dXn[:, 1])
-        I_n = np.eye(n)
-
-        def dxi(A):
-            return _tensor3(DCLn, I_n, I_n, A)
-
-        def deta(A):
-            return _tensor3(I_n, DCLn, I_n, A)
-
-        def dzeta(A):
-            return _tensor3(I_n, I_n, DCLn, A)
-
+        Rc[:, :, 0] = 0.5 * (Xc[:, :, 2] * dXn[:, :, 1] - Xc[:, :, 1] * dXn[:, :, 2])
+        Rc[:, :, 1] = 0.5 * (Xc[:, :, 0] * dXn[:, :, 2] - Xc[:, :, 2] * dXn[:, :, 0])
+        Rc[:, :, 2] = 0.5 * (Xc[:, :, 1] * dXn[:, :, 0] - Xc[:, :, 0] * dXn[:, :, 1])
         JaCL = np.empty_like(dXn)
-        JaCL[0] = -deta(Rc[2]) + dzeta(Rc[1])
-        JaCL[1] = -dzeta(Rc[0]) + dxi(Rc[2])
-        JaCL[2] = -dxi(Rc[1]) + deta(Rc[0])
-        Ja = np.stack([_tensor3(V_CLn_N, V_CLn_N, V_CLn_N, JaCL[d]) for d in range(3)], axis=0)
-        xGP = _tensor3(V_CLg_N, V_CLg_N, V_CLg_N, XCLg)
-        return Ja, sJ, JaCL, xGP
+        JaCL[:, 0] = -_along_j(DCLn, Rc[:, 2]) + _along_k(DCLn, Rc[:, 1])
+        JaCL[:, 1] = -_along_k(DCLn, Rc[:, 0]) + _along_i(DCLn, Rc[:, 2])
+        JaCL[:, 2] = -_along_i(DCLn, Rc[:, 1]) + _along_j(DCLn, Rc[:, 0])
+        Ja = _tensor3(V_CLn_N, V_CLn_N, V_CLn_N, JaCL)
+        xg = _tensor3(V_CLg_N, V_CLg_N, V_CLg_N, XCLg)
+        return Ja, sJe, JaCL, xg
 
     mf = np.zeros((3, n, n, n, nE), order="F")
     mg = np.zeros((3, n, n, n, nE), order="F")
@@ -305,75 +308,76 @@ def box_mesh(nelems, N, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), deform_amp=0.0, 
     t2 = np.zeros((3, n, n, nS), order="F")
     se = np.zeros((n, n, nS), order="F")
 
-    def face_metrics(JaCL_e, ls):
-        """Ja on the master face in side (p,q) order (metrics.f90:700-740)."""
-        sel = {XI_MINUS: (slice(None), slice(None), 0), XI_PLUS: (slice(None), slice(None), N)}
+    def face_metrics(JaCL_m, ls):
+        """Ja on the master faces of local side ls in side (p,q) order (metrics.f90:700-740).
+        JaCL_m(m, dir, comp, i, j, k) -> (dir, comp, p, q, m)"""
         if ls == XI_MINUS:
-            F = JaCL_e[:, :, 0, :, :]
+            F = JaCL_m[:, :, :, 0, :, :]
         elif ls == XI_PLUS:
-            F = JaCL_e[:, :, N, :, :]
+            F = JaCL_m[:, :, :, N, :, :]
         elif ls == ETA_MINUS:
-            F = JaCL_e[:, :, :, 0, :]
+            F = JaCL_m[:, :, :, :, 0, :]
         elif ls == ETA_PLUS:
-            F = JaCL_e[:, :, :, N, :]
+            F = JaCL_m[:, :, :, :, N, :]
         elif ls == ZETA_MINUS:
-            F = JaCL_e[:, :, :, :, 0]
+            F = JaCL_m[:, :, :, :, :, 0]
         else:
-            F = JaCL_e[:, :, :, :, N]
-        del sel
-        F = np.einsum("ai,bj,dcij->dcab", V_CLn_N, V_CLn_N, F)
-        out = np.empty_like(F)
-        for q in range(n):
-            for p in range(n):
-                a, b = s2v2[0, p, q, 0, ls - 1], s2v2[1, p, q, 0, ls - 1]
-                out[:, :, p, q] = F[:, :, a, b]
-        return out
+            F = JaCL_m[:, :, :, :, :, N]
+        F = np.matmul(np.matmul(V_CLn_N, F), V_CLn_N.T)  # (m, dir, comp, a, b)
+        A, B = s2v2[0, :, :, 0, ls - 1], s2v2[1, :, :, 0, ls - 1]
+        return np.moveaxis(F[..., A, B], 0, -1)
 
-    master_sides = [(int(s2e[0, s] - 1), int(s2e[2, s]), s) for s in range(nS)]
+    ls_of_side = s2e[2, :]
     if uniform:
         Ja1, sJ1, JaCL1, _x = geometry(np.array([0]), centre_override=np.zeros(3))
-        mf[:] = Ja1[0][..., :1]
-        mg[:] = Ja1[1][..., :1]
-        mh[:] = Ja1[2][..., :1]
-        sJ[:] = sJ1[..., :1]
+        mf[:] = Ja1[0, 0][..., None]
+        mg[:] = Ja1[0, 1][..., None]
+        mh[:] = Ja1[0, 2][..., None]
+        sJ[:] = sJ1[0][..., None]
         xi3 = np.stack(np.meshgrid(xGL, xGL, xGL, indexing="ij"), axis=0)
         centre = (lo[:, None] + (cells.T + 0.5) * h[:, None])
         xGP[:] = centre[:, None, None, None, :] + 0.5 * h[:, None, None, None, None] * xi3[..., None]
-        tmpl = {}
-        for ls in range(1, 7):
-            JaF = face_metrics(JaCL1[..., 0], ls)
-            tmpl[ls] = _surf_metrics_from_ja(NORMAL_DIRS[ls - 1], TANG_DIRS[ls - 1], NORMAL_SIGNS[ls - 1], JaF)
-        ls_of_side = s2e[2, :]
         for ls in range(1, 7):
             ids = np.nonzero(ls_of_side == ls)[0]
             if len(ids) == 0:
                 continue
-            a, b, c, d = tmpl[ls]
-            nv[:, :, :, ids] = a[..., None]
-            t1[:, :, :, ids] = b[..., None]
-            t2[:, :, :, ids] = c[..., None]
-            se[:, :, ids] = d[..., None]
+            a, b, c, d = _surf_metrics_from_ja(NORMAL_DIRS[ls - 1], TANG_DIRS[ls - 1], NORMAL_SIGNS[ls - 1],
+                                               face_metrics(JaCL1, ls))
+            nv[:, :, :, ids] = a
+            t1[:, :, :, ids] = b
+            t2[:, :, :, ids] = c
+            se[:, :, ids] = d
     else:
-        side_of_master = {}
-        for e, ls, s in master_sides:
-            side_of_master.setdefault(e, []).append((ls, s))
-        for c0 in range(0, nE, chunk):
+        master_elem = s2e[0, :] - 1
+
+        def fill(c0):
             ids = np.arange(c0, min(nE, c0 + chunk))
             Ja, sJc, JaCL, xg = geometry(ids)
-            mf[..., ids] = Ja[0]
-            mg[..., ids] = Ja[1]
-            mh[..., ids] = Ja[2]
-            sJ[..., ids] = sJc
-            xGP[..., ids] = xg
-            for li, e in enumerate(ids):
-                for ls, s in side_of_master.get(int(e), []):
-                    JaF = face_metrics(JaCL[..., li], ls)
-                    a, b, c, d = _surf_metrics_from_ja(NORMAL_DIRS[ls - 1], TANG_DIRS[ls - 1],
-                                                       NORMAL_SIGNS[ls - 1], JaF)
-                    nv[:, :, :, s] = a
-                    t1[:, :, :, s] = b
-                    t2[:, :, :, s] = c
-                    se[:, :, s] = d
+            mf[..., ids] = np.moveaxis(Ja[:, 0], 0, -1)
+            mg[..., ids] = np.moveaxis(Ja[:, 1], 0, -1)
+            mh[..., ids] = np.moveaxis(Ja[:, 2], 0, -1)
+            sJ[..., ids] = np.moveaxis(sJc, 0, -1)
+            xGP[..., ids] = np.moveaxis(xg, 0, -1)
+            in_chunk = (master_elem >= c0) & (master_elem < c0 + len(ids))
+            for ls in range(1, 7):
+                sides = np.nonzero(in_chunk & (ls_of_side == ls))[0]
+                if len(sides) == 0:
+                    continue
+                JaF = face_metrics(JaCL[master_elem[sides] - c0], ls)
+                a, b, c, d = _surf_metrics_from_ja(NORMAL_DIRS[ls - 1], TANG_DIRS[ls - 1], NORMAL_SIGNS[ls - 1], JaF)
+                nv[:, :, :, sides] = a
+                t1[:, :, :, sides] = b
+                t2[:, :, :, sides] = c
+                se[:, :, sides] = d
+
+        # chunks are independent and write disjoint slices; numpy releases the GIL in matmul and ufuncs
+        workers = max(1, min(threads or os.cpu_count() or 1, 16, (nE + chunk - 1) // chunk))
+        if workers == 1:
+            for c0 in range(0, nE, chunk):
+                fill(c0)
+        else:
+            with ThreadPoolExecutor(workers) as ex:
+                list(ex.map(fill, range(0, nE, chunk)))
     return Mesh(N=N, nElems=nE, nSides=nS, ElemToSide=e2s, SideToElem=s2e, S2V2=s2v2,
                 Metrics_fTilde=mf, Metrics_gTilde=mg, Metrics_hTilde=mh, sJ=sJ, NormVec=nv, TangVec1=t1,
                 TangVec2=t2, SurfElem=se, Elem_xGP=xGP, lo=tuple(lo), hi=tuple(hi),

@@ -65,3 +65,43 @@ def test_null_context_is_an_argument_error(L):
     assert L.galx_rhs_vjp(None, None, None) == -1
     assert L.galx_lserk_adjoint(None, None) == -1
     assert L.galx_destroy(None) == 0
+
+
+def test_fortran_binding_compiles_and_matches_header(tmp_path):
+    """include/galx_mod.f90 (the ISO_C_BINDING module for a Fortran host) compiles with ROCm flang, names only
+    functions the C header declares, and a program using it links against the library and gets GALX_E_ARG for
+    a bad configuration."""
+    import shutil
+    import subprocess
+
+    src = os.path.join(ROOT, "include", "galx_mod.f90")
+    names = set(re.findall(r"NAME='(galx_[a-z_0-9]+)'", open(src).read()))
+    assert names and names <= set(header_functions())
+    flang = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
+    if not os.path.exists(flang):
+        pytest.skip("flang not available")
+    prog = tmp_path / "use_galx.f90"
+    prog.write_text("""PROGRAM use_galx
+USE ISO_C_BINDING
+USE MOD_GALX
+IMPLICIT NONE
+TYPE(galx_config) :: cfg
+TYPE(C_PTR) :: ctx
+INTEGER(C_INT) :: rc
+cfg = galx_config(7, GALX_FORM_SPLIT_PI, GALX_RIEMANN_LF, GALX_PREC_FP64, 1.4d0, 71.42857d0, 0.72d0, &
+                  1.d0/1600.d0, 0.5d0, 0.5d0, 0)
+cfg%form = 9
+rc = galx_create(cfg, ctx)
+PRINT *, rc
+IF (rc .NE. -1) STOP 1
+END PROGRAM use_galx
+""")
+    r = subprocess.run([flang, "-c", src, "-o", str(tmp_path / "galx_mod.o"), "-J", str(tmp_path)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    r = subprocess.run([flang, str(prog), str(tmp_path / "galx_mod.o"), "-I", str(tmp_path), "-L", libdir, "-lgalx",
+                        f"-Wl,-rpath,{libdir}", "-o", str(tmp_path / "use_galx")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(tmp_path / "use_galx")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.split()[0] == "-1", (r.stdout, r.stderr)

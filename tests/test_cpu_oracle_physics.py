@@ -128,3 +128,63 @@ def test_calc_dt_err_type():
     U[0, 0, 0, 0, 1] = -0.5
     _dt, err, _c, _v = O.Operator(m).calc_dt(U)
     assert err == 1
+
+
+def _normal_shock(M1=2.0, rho1=1.0, p1=1.0, kappa=1.4, vt=(0.3, -0.2)):
+    """Stationary normal shock in +x (Rankine-Hugoniot), tangential velocity continuous."""
+    c1 = np.sqrt(kappa * p1 / rho1)
+    u1 = M1 * c1
+    rho2 = rho1 * (kappa + 1) * M1 ** 2 / ((kappa - 1) * M1 ** 2 + 2)
+    p2 = p1 * (1 + 2 * kappa / (kappa + 1) * (M1 ** 2 - 1))
+    u2 = u1 * rho1 / rho2
+    return (rho1, u1, vt[0], vt[1], p1), (rho2, u2, vt[0], vt[1], p2)
+
+
+def _euler_flux_n(prim, nv, kappa=1.4):
+    rho, u, v, w, p = prim
+    vel = np.array([u, v, w])
+    vn = vel @ nv
+    E = p / (kappa - 1) + 0.5 * rho * vel @ vel
+    return np.array([rho * vn, *(rho * vn * vel + p * nv), vn * (E + p)])
+
+
+@pytest.mark.parametrize("riemann", [O.RIEMANN_ROE, O.RIEMANN_ROEEF, O.RIEMANN_LF])
+def test_riemann_flux_at_stationary_shock(riemann):
+    """Elements with x-index 0 hold the upstream state of a Mach-2 stationary shock, x-index 1 the downstream
+    state (inviscid, weak form).  Faces between equal states give F(U).n (consistency, all solvers).  Roe's
+    linearisation resolves any Rankine-Hugoniot pair exactly, so both x-face families give F(U_up).n; the
+    entropy fix stays inactive at the admissible (upstream-master) faces, so RoeEntropyFix is exact there; the
+    Lax-Friedrichs flux carries its dissipation -lmax/2 (U_R - U_L) with lmax = max|u_n| + max c."""
+    up, dn = _normal_shock()
+    m = M.box_mesh((2, 2, 2), 2, lo=(0, 0, 0), hi=(1, 1, 1))
+    xi = (m.Elem_xGP[0].reshape(-1, m.nElems, order="F").mean(axis=0) > 0.5).astype(int)
+    prims = [np.where(xi[None, None, None, :] == 0, a, b) * np.ones(m.sJ.shape) for a, b in zip(up, dn)]
+    U = T.prim_to_cons(*prims, 1.4)
+    op = O.Operator(m, split=False, riemann=riemann, mu0=0.0, R=1.0)
+    op.rhs(U)
+    Fm = op.array(8).reshape((5, 3, 3, m.nSides), order="F")
+    checked = {"same": 0, "up": 0, "down": 0}
+    for s in range(m.nSides):
+        e_m, e_s = m.SideToElem[0, s] - 1, m.SideToElem[1, s] - 1
+        nv = m.NormVec[:, 0, 0, s]
+        se = m.SurfElem[0, 0, s]
+        F = Fm[:, 1, 1, s] / se
+        if xi[e_m] == xi[e_s]:
+            exact = _euler_flux_n(up if xi[e_m] == 0 else dn, nv)
+            assert np.allclose(F, exact, rtol=1e-13, atol=1e-13)
+            checked["same"] += 1
+            continue
+        admissible = xi[e_m] == 0 and nv[0] > 0.5
+        exact = _euler_flux_n(up, nv)
+        assert np.allclose(exact, _euler_flux_n(dn, nv), rtol=1e-13)
+        if riemann == O.RIEMANN_ROE or (riemann == O.RIEMANN_ROEEF and admissible):
+            assert np.allclose(F, exact, rtol=1e-12, atol=1e-12), (riemann, s, F, exact)
+        elif riemann == O.RIEMANN_LF:
+            L, Rr = (up, dn) if xi[e_m] == 0 else (dn, up)
+            vnL, vnR = np.array(L[1:4]) @ nv, np.array(Rr[1:4]) @ nv
+            lmax = max(abs(vnL), abs(vnR)) + max(np.sqrt(1.4 * L[4] / L[0]), np.sqrt(1.4 * Rr[4] / Rr[0]))
+            UL = T.prim_to_cons(*[np.array([v]) for v in L], 1.4)[:, 0]
+            UR = T.prim_to_cons(*[np.array([v]) for v in Rr], 1.4)[:, 0]
+            assert np.allclose(F, exact - 0.5 * lmax * (UR - UL), rtol=1e-12, atol=1e-12)
+        checked["up" if admissible else "down"] += 1
+    assert checked["same"] > 0 and checked["up"] > 0 and checked["down"] > 0

@@ -183,15 +183,15 @@ C4_GAS = dict(kappa=1.4, R=1.0, Pr=0.72, mu0=1e-3)
 
 def c4_case(kind):
     """16^3 elements, N=5, curved periodic box.  'smooth': [-1,1]^3 with the 0.05 sine mapping and the seeded
-    smooth state (traces continuous across faces); 'tgv': [0,2pi]^3 with the mapping scaled by pi, Taylor-Green
-    state with 1e-2 nodal velocity noise (velocity jumps, density and pressure continuous: the LF sound-speed
-    switch sits on its tie at every face node)."""
+    smooth state (traces continuous across faces); 'tgv': [0,2pi]^3 with the same mapping scaled to the box,
+    Taylor-Green state with 1e-2 nodal velocity noise (velocity jumps, density and pressure continuous: the LF
+    sound-speed switch sits on its tie at every face node)."""
     if kind == "smooth":
         lo, hi = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
         m = M.box_mesh((16, 16, 16), 5, lo=lo, hi=hi, deform_amp=0.05, deform_phase=(0.31, 0.17, 0.83))
         return m, T.smooth_random(m.Elem_xGP, lo, hi, seed=44), C4_GAS
     lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
-    m = M.box_mesh((16, 16, 16), 5, lo=lo, hi=hi, deform_amp=0.05 * np.pi, deform_phase=(0.1, 0.2, 0.3))
+    m = M.box_mesh((16, 16, 16), 5, lo=lo, hi=hi, deform_amp=0.05, deform_phase=(0.1, 0.2, 0.3))
     return m, T.taylor_green(m.Elem_xGP, Ma=0.1, noise=1e-2, seed=16), dict(kappa=1.4, R=71.42857, Pr=0.72,
                                                                               mu0=1.0 / 1600.0)
 
