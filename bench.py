@@ -283,7 +283,7 @@ def main(argv=None):
     if a.adjoint_ne:
         print(f"# adjoint: curved TGV N={N} {a.adjoint_ne}^3, {a.adjoint_steps} steps, checkpoints every "
               f"{a.adjoint_ckpt}", file=sys.stderr, flush=True)
-        out["adjoint"] = run_adjoint(a.adjoint_ne, a.adjoint_steps, a.adjoint_ckpt, (1e-5, 1e-6))
+        out["adjoint"] = run_adjoint(a.adjoint_ne, a.adjoint_steps, a.adjoint_ckpt, (1e-4, 3e-5, 1e-5))
     if not a.no_cpu_baseline:
         print("# cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline()

@@ -143,6 +143,23 @@ __global__ void k_planes_to_ref(long nDOF, int n3, const double* __restrict__ sr
     for (int v = 0; v < 5; v++) dst[5 * x + v] = src[(e * 5 + v) * n3 + node];
 }
 
+// gradient planes [e][12][n3] -> gradUx/y/z(4,0:N,0:N,0:N,nElems) (lifting variables u,v,w,T)
+__global__ void k_grad_to_ref(long nDOF, int n3, const double* __restrict__ g, double* __restrict__ gx,
+                              double* __restrict__ gy, double* __restrict__ gz)
+{
+    long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nDOF) return;
+    long e = x / n3;
+    int node = (int)(x - e * n3);
+    const double* s = g + e * 12 * n3 + node;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        gx[4 * x + q] = s[q * n3];
+        gy[4 * x + q] = s[(4 + q) * n3];
+        gz[4 * x + q] = s[(8 + q) * n3];
+    }
+}
+
 // sum of the pressure over all nodes (element planes), for the pressure offset of the mixed form
 __global__ void __launch_bounds__(256) k_pressure_sum(long nDOF, int n3, double kappa, const double* __restrict__ U,
                                                       double* __restrict__ out)
@@ -176,6 +193,8 @@ struct KernelSet {
                 int, double, double, Stage) = nullptr;
     void (*dt)(hipStream_t, int, const double*, const double*, double, double, double, double, double, double*,
                double*, int*) = nullptr;
+    void (*grad)(hipStream_t, int, const double*, Mesh, const void*, const void*, GasD, double, double,
+                 double*) = nullptr;
 };
 
 template <int N, typename T, bool SPLIT>
@@ -214,9 +233,16 @@ struct Launch {
         hipLaunchKernelGGL((k_dt_elem<N>), dim3(nE), dim3(D::block), 0, s, U, geo, kappa, mu0, KPmax, CFL, DFL, dtc,
                            dtv, err);
     }
+    static void grad(hipStream_t s, int nE, const double* U, Mesh m, const void* utr, const void* mats, GasD g,
+                     double LHm, double LHp, double* gvol)
+    {
+        hipLaunchKernelGGL((k_grad<N, T, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr, (const T*)mats, g,
+                           T(LHm), T(LHp), gvol);
+    }
     static KernelSet make()
     {
         KernelSet k;
+        k.grad = &grad;
         k.ok = true;
         k.block = D::block;
         k.trace = &trace;
@@ -336,6 +362,7 @@ struct galx_ctx {
     double trajT = 0.0, trajDt = 0.0;
     int trajSteps = 0, trajEvery = 0;
     bool trajValid = false;
+    double *dGrad = nullptr, *dGradRef = nullptr;  // BR1 gradients for analysis (planes, reference layout)
     long long bytes = 0;
     bool timing = false;
     std::map<std::string, Timer> timers;
@@ -487,7 +514,7 @@ int galx_destroy(galx_ctx* c)
     void* ps[] = {c->dU, c->dK, c->dUt, c->dScratch, c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dUtr[0], c->dUtr[1],
                   c->dGtr, c->dMats, c->dDtc, c->dDtv, c->dDtOut, c->dErr};
     for (void* p : ps) dfree(p);
-    void* pa[] = {c->dKbar, c->dUbar, c->dMutr, c->dNghtr, c->dUfinal};
+    void* pa[] = {c->dKbar, c->dUbar, c->dMutr, c->dNghtr, c->dUfinal, c->dGrad, c->dGradRef};
     for (void* p : pa) dfree(p);
     for (double* p : c->ckpt) dfree(p);
     for (double* p : c->stages) dfree(p);
@@ -779,6 +806,65 @@ int galx_lserk_steps(galx_ctx* c, double t, double dt, int nSteps)
 }
 
 int galx_lserk_step(galx_ctx* c, double t, double dt) { return galx_lserk_steps(c, t, dt, 1); }
+
+int galx_lserk_stage(galx_ctx* c, double t, double dt, int iStage)
+{
+    (void)t;
+    if (!c) return fail(GALX_E_ARG, "null context");
+    if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
+    if (iStage < 1 || iStage > 5 || !(dt == dt)) return fail(GALX_E_ARG, "bad stage arguments");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    int stage = iStage - 1;
+    launch_lift(c);
+    double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
+    launch_rhs(c, Stage{mRKA, RKB[stage] * dt, 1});
+    c->cur ^= 1;
+    HIPCHK(hipGetLastError());
+    c->utValid = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+static int grad_enqueue(galx_ctx* c, double* gx, double* gy, double* gz)
+{
+    long nDOF = (long)c->nElems * c->n3;
+    int rc;
+    if ((rc = dalloc(c, &c->dGrad, 12 * nDOF * sizeof(double)))) return rc;
+    const Basis& B = c->basis;
+    c->ks.grad(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[c->cur], c->dMats, gas_of(c), 1.0 / B.w[0],
+               1.0 / B.w[c->N], c->dGrad);
+    k_grad_to_ref<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dGrad, gx, gy, gz);
+    HIPCHK(hipGetLastError());
+    return GALX_OK;
+}
+
+int galx_get_gradients_device(galx_ctx* c, double* dGx, double* dGy, double* dGz)
+{
+    if (!c || !dGx || !dGy || !dGz) return fail(GALX_E_ARG, "null argument");
+    if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    int rc = grad_enqueue(c, dGx, dGy, dGz);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+int galx_get_gradients(galx_ctx* c, double* gx, double* gy, double* gz)
+{
+    if (!c || !gx || !gy || !gz) return fail(GALX_E_ARG, "null argument");
+    if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    long nDOF = (long)c->nElems * c->n3;
+    int rc;
+    if ((rc = dalloc(c, &c->dGradRef, 12 * nDOF * sizeof(double)))) return rc;
+    double *dx = c->dGradRef, *dy = dx + 4 * nDOF, *dz = dy + 4 * nDOF;
+    if ((rc = grad_enqueue(c, dx, dy, dz))) return rc;
+    HIPCHK(hipMemcpyAsync(gx, dx, 4 * nDOF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(gy, dy, 4 * nDOF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(gz, dz, 4 * nDOF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
 
 // CFLScaleAlpha / DFLScaleAlpha for Gauss-Lobatto nodes, Carpenter RK4-5 (src/timedisc/timedisc_vars.f90:105-109,
 // 188-227); scaling as FillCFL_DFL (src/timedisc/timedisc_func.f90:480-550)

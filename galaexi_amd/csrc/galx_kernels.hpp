@@ -318,6 +318,40 @@ __global__ void __launch_bounds__(Dm<N>::block)
 }
 
 // ---------------------------------------------------------------------------------------------------------
+// k_grad: the BR1 volume gradients of (u,v,w,T) of the current state, as the reference keeps them in
+// gradUx/y/z (lifting_br1.t90:47-167 with PP_OPTLIFT), for analysis: gvol [e][12][n3] in FP64
+// ---------------------------------------------------------------------------------------------------------
+template <int N, typename T, typename TT>
+__global__ void __launch_bounds__(Dm<N>::block)
+    k_grad(const double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, const T* __restrict__ mats, GasD gd,
+           T LHm, T LHp, double* __restrict__ gvol)
+{
+    using D = Dm<N>;
+    constexpr int n2 = D::n2, n3 = D::n3;
+    __shared__ T s_buf[4 * n3 + 6 * 7 * n2];
+    __shared__ Mats<N, T> M;
+    const int e = blockIdx.x, tid = threadIdx.x;
+    load_mats<N, T>(M, mats, tid, D::block);
+    const Gas<T> gas = make_gas<T>(gd);
+    T uc[5], pr[6], H;
+    if (tid < n3) {
+        load_state<N, T>(e, tid, U, gd, uc, pr, H);
+        s_buf[tid] = pr[1];
+        s_buf[n3 + tid] = pr[2];
+        s_buf[2 * n3 + tid] = pr[3];
+        s_buf[3 * n3 + tid] = pr[5];
+    }
+    __syncthreads();
+    T g[12];
+    lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_buf, s_buf + 4 * n3, gas, LHm, LHp, g);
+    if (tid < n3) {
+        double* o = gvol + (long)e * 12 * n3 + tid;
+#pragma unroll
+        for (int q = 0; q < 12; q++) o[q * n3] = double(g[q]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------
 // k_rhs: DGTimeDerivative_weakForm (src/dg/dg.f90:255-425) after the lifting, fused with the LSERK register
 // update VAXPB_OUT_VAXPB_IN (src/globals/vector.f90:163-183, src/timedisc/timestep.f90:85-119).
 //   SPLIT: VolInt_weakForm_Visc + VolInt_splitForm_Kernel (src/dg/volint.f90:60-119,265-353)

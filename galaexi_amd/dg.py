@@ -115,6 +115,18 @@ class DGOperator:
         check(lib().galx_lserk_steps(self._h, float(t), float(dt), int(nSteps)), "galx_lserk_steps")
         return t + nSteps * dt
 
+    def lserk_stage(self, t, dt, iStage):
+        """Stage iStage (1..5) of TimeStepByLSERKW2."""
+        check(lib().galx_lserk_stage(self._h, float(t), float(dt), int(iStage)), "galx_lserk_stage")
+
+    def get_gradients(self):
+        """Lifted gradients (gradUx, gradUy, gradUz) of the current state, each (4,0:N,0:N,0:N,nElems) over the
+        lifting variables (u,v,w,T)."""
+        n = self.n
+        out = [np.zeros((4, n, n, n, self.nElems), order="F") for _ in range(3)]
+        check(lib().galx_get_gradients(self._h, *[_dptr(a) for a in out]), "galx_get_gradients")
+        return tuple(out)
+
     # CALCTIMESTEP
     def CalcTimeStep(self):
         dt, err = C.c_double(), C.c_int()

@@ -97,6 +97,16 @@ int galx_lserk_step(galx_ctx *ctx, double t, double dt);
 /* nSteps steps with a fixed dt, without host synchronisation between steps. */
 int galx_lserk_steps(galx_ctx *ctx, double t, double dt, int nSteps);
 
+/* One stage iStage = 1..5 of TimeStepByLSERKW2 (src/timedisc/timestep.f90:85-119, CurrentStage = iStage); five
+ * calls in order equal one galx_lserk_step.  Lets the host act between stages as the reference's loop does. */
+int galx_lserk_stage(galx_ctx *ctx, double t, double dt, int iStage);
+
+/* Lifted gradients of the current state, as MOD_Lifting_Vars holds them after DGTimeDerivative_weakForm
+ * (src/dg/lifting/lifting_br1.t90:47-167) with the PP_OPTLIFT lifting variables (u,v,w,T)
+ * (src/equations/navierstokes/idealgas/eos.h:145-152): gradUx/y/z(4,0:N,0:N,0:N,nElems), FP64, host / device. */
+int galx_get_gradients(galx_ctx *ctx, double *gradUx, double *gradUy, double *gradUz);
+int galx_get_gradients_device(galx_ctx *ctx, double *dGradUx, double *dGradUy, double *dGradUz);
+
 /* CALCTIMESTEP(errType) (src/equations/navierstokes/calctimestep.f90:98-186), FP64 reduction.
  * errType: 0 ok, 1 non-positive or NaN density, 2 convective / 3 viscous time step is NaN. */
 int galx_calc_dt(galx_ctx *ctx, double *dt, int *errType);
