@@ -204,28 +204,28 @@ struct Launch {
     {
         hipLaunchKernelGGL((k_trace<N, T>), dim3(nE), dim3(D::block), 0, s, U, m, (T*)utr, g);
     }
-    static void lift(hipStream_t s, int nE, const double* U, Mesh m, const void* utr, void* gtr, const void* mats,
+    static void lift(hipStream_t s, int nE, const double* U, Mesh m, const void* utr, void* vtr, const void* mats,
                      GasD g, double LHm, double LHp)
     {
-        hipLaunchKernelGGL((k_lift<N, T, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr, (T*)gtr,
+        hipLaunchKernelGGL((k_lift<N, T, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr, (T*)vtr,
                            (const T*)mats, g, T(LHm), T(LHp));
     }
     template <int RIE>
-    static void rhs_r(hipStream_t s, int nE, double* U, Mesh m, const void* utr, const void* gtr, void* utr_next,
+    static void rhs_r(hipStream_t s, int nE, double* U, Mesh m, const void* utr, const void* vtr, void* utr_next,
                       double* Ut, double* K, const void* mats, GasD g, double LHm, double LHp, Stage st)
     {
         hipLaunchKernelGGL((k_rhs<N, T, T, SPLIT, RIE>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr,
-                           (const T*)gtr, (T*)utr_next, Ut, K, (const T*)mats, g, T(LHm), T(LHp), st);
+                           (const T*)vtr, (T*)utr_next, Ut, K, (const T*)mats, g, T(LHm), T(LHp), st);
     }
-    static void rhs(hipStream_t s, int nE, double* U, Mesh m, const void* utr, const void* gtr, void* utr_next,
+    static void rhs(hipStream_t s, int nE, double* U, Mesh m, const void* utr, const void* vtr, void* utr_next,
                     double* Ut, double* K, const void* mats, GasD g, int riemann, double LHm, double LHp, Stage st)
     {
         if (riemann == GALX_RIEMANN_LF)
-            rhs_r<RIEMANN_LF>(s, nE, U, m, utr, gtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
+            rhs_r<RIEMANN_LF>(s, nE, U, m, utr, vtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
         else if (riemann == GALX_RIEMANN_ROE)
-            rhs_r<RIEMANN_ROE>(s, nE, U, m, utr, gtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
+            rhs_r<RIEMANN_ROE>(s, nE, U, m, utr, vtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
         else
-            rhs_r<RIEMANN_ROEEF>(s, nE, U, m, utr, gtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
+            rhs_r<RIEMANN_ROEEF>(s, nE, U, m, utr, vtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
     }
     static void dt(hipStream_t s, int nE, const double* U, const double* geo, double kappa, double mu0, double KPmax,
                    double CFL, double DFL, double* dtc, double* dtv, int* err)
@@ -348,7 +348,7 @@ struct galx_ctx {
     double *dU = nullptr, *dK = nullptr, *dUt = nullptr, *dScratch = nullptr;
     double *dGeo = nullptr, *dFgeo = nullptr;
     int *dConn = nullptr, *dInv = nullptr;
-    void *dUtr[2] = {nullptr, nullptr}, *dGtr = nullptr, *dMats = nullptr;
+    void *dUtr[2] = {nullptr, nullptr}, *dVtr = nullptr, *dMats = nullptr;
     double *dDtc = nullptr, *dDtv = nullptr, *dDtOut = nullptr;
     int* dErr = nullptr;
     int cur = 0;
@@ -512,7 +512,7 @@ int galx_destroy(galx_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     flush_timers(c);
     void* ps[] = {c->dU, c->dK, c->dUt, c->dScratch, c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dUtr[0], c->dUtr[1],
-                  c->dGtr, c->dMats, c->dDtc, c->dDtv, c->dDtOut, c->dErr};
+                  c->dVtr, c->dMats, c->dDtc, c->dDtv, c->dDtOut, c->dErr};
     for (void* p : ps) dfree(p);
     void* pa[] = {c->dKbar, c->dUbar, c->dMutr, c->dNghtr, c->dUfinal, c->dGrad, c->dGradRef};
     for (void* p : pa) dfree(p);
@@ -580,7 +580,7 @@ int galx_set_mesh(galx_ctx* c, int nElems, int nSides, const int* E2S, const int
     if ((rc = dalloc(c, &c->dInv, inv.size() * sizeof(int)))) return rc;
     if ((rc = dalloc(c, (char**)&c->dUtr[0], 2 * 5 * nfn * c->tsz))) return rc;
     if ((rc = dalloc(c, (char**)&c->dUtr[1], 2 * 5 * nfn * c->tsz))) return rc;
-    if ((rc = dalloc(c, (char**)&c->dGtr, 2 * 12 * nfn * c->tsz))) return rc;
+    if ((rc = dalloc(c, (char**)&c->dVtr, 2 * 4 * nfn * c->tsz))) return rc;
     if ((rc = dalloc(c, &c->dDtc, nElems * sizeof(double)))) return rc;
     if ((rc = dalloc(c, &c->dDtv, nElems * sizeof(double)))) return rc;
     if ((rc = dalloc(c, &c->dDtOut, 4 * sizeof(double)))) return rc;
@@ -715,7 +715,7 @@ static void launch_lift(galx_ctx* c)
 {
     TimerScope ts(c, "lift");
     const Basis& B = c->basis;
-    c->ks.lift(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[c->cur], c->dGtr, c->dMats, gas_of(c), 1.0 / B.w[0],
+    c->ks.lift(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[c->cur], c->dVtr, c->dMats, gas_of(c), 1.0 / B.w[0],
                1.0 / B.w[c->N]);
 }
 
@@ -723,7 +723,7 @@ static void launch_rhs(galx_ctx* c, Stage st)
 {
     TimerScope ts(c, "rhs");
     const Basis& B = c->basis;
-    c->ks.rhs(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[c->cur], c->dGtr, c->dUtr[c->cur ^ 1], c->dUt, c->dK,
+    c->ks.rhs(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[c->cur], c->dVtr, c->dUtr[c->cur ^ 1], c->dUt, c->dK,
               c->dMats, gas_of(c), c->cfg.riemann, 1.0 / B.w[0], 1.0 / B.w[c->N], st);
 }
 

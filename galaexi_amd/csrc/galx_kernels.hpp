@@ -5,14 +5,17 @@
 //   geo           : double [nElems][10][n^3]         (Ja^1(3), Ja^2(3), Ja^3(3), sJ)
 //   fgeo          : double [nSides][10][n^2]         (NormVec(3), TangVec1(3), TangVec2(3), SurfElem), master (p,q)
 //   utr[2]        : TT [nSides][2][5][n^2]           U traces, slot 0 master / 1 slave, master (p,q); double buffered
-//   gtr           : TT [nSides][2][12][n^2]          lifted gradient traces (u,v,w,T) x (x,y,z)
+//   vtr           : TT [nSides][2][4][n^2]           viscous normal-flux traces F_v(U,grad U).n_master, components
+//                                                    (rho u, rho v, rho w, E) of each side (the mass component is 0)
 //   conn          : int [nElems][6]                  (SideID-1) << 3 | flip
 //   inv           : int [5 flips][6 locSides][n^2]   S2V2_inv as p + n*q
 // TT (trace type) is double in the FP64 form and float in the mixed form.
 //
-// Per RK stage: k_lift (BR1 gradients -> gradient traces) then k_rhs (gradients recomputed, viscous weak-form
-// volume term, split-form flux differencing or weak Euler term, Riemann + viscous face fluxes, surface integral,
-// -sJ scaling and the LSERK register update; writes U, Ut_tmp and the next U traces).
+// Per RK stage: k_lift (BR1 gradients -> viscous normal-flux traces) then k_rhs (gradients recomputed, viscous
+// weak-form volume term, split-form flux differencing or weak Euler term, Riemann + viscous face fluxes, surface
+// integral, -sJ scaling and the LSERK register update; writes U, Ut_tmp and the next U traces).  The BR1 face flux
+// 0.5*(F_v(master) + F_v(slave)).n (riemann.f90:638-700) needs only the normal viscous flux of each side, so the
+// sides exchange those 4 values per face node instead of the 12 gradient components.
 #pragma once
 #include "galx_physics.hpp"
 
@@ -234,6 +237,31 @@ __device__ __forceinline__ void store_trace(TT* o, int n2, const double* c, cons
     for (int v = 0; v < 5; v++) o[v * n2] = TT(tr[v]);
 }
 
+// normal viscous flux F_v(P, g).n_master of node (i,j,k) on each of its faces -> s_fv[6][4][n2] (components
+// rho u, rho v, rho w, E; EvalNormalDiffFlux3D flux.f90:779-847)
+template <int N, typename T>
+__device__ __forceinline__ void store_face_viscous(int e, int i, int j, int k, const Mesh& mesh, const T* pr,
+                                                   const T* g, const Gas<T>& gas, T* s_fv)
+{
+    constexpr int n2 = (N + 1) * (N + 1);
+#pragma unroll
+    for (int ls = 1; ls <= 6; ls++) {
+        int ab;
+        if (!on_face<N>(ls, i, j, k, ab)) continue;
+        int c = mesh.conn[e * 6 + ls - 1];
+        long side = c >> 3;
+        int flip = c & 7;
+        int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
+        const double* fg = mesh.fgeo + side * 10 * n2 + pq;
+        T nv[3] = {T(fg[0]), T(fg[n2]), T(fg[2 * n2])};
+        T f[5];
+        normal_diff_flux(pr, g, nv, gas, f);
+        T* o = s_fv + (ls - 1) * 4 * n2 + ab;
+#pragma unroll
+        for (int q = 0; q < 4; q++) o[q * n2] = f[1 + q];
+    }
+}
+
 // ---------------------------------------------------------------------------------------------------------
 // k_trace: U traces of all faces (ProlongToFaceCons, EvalElemFaceGL_Device prolongtoface.t90:268-344, in the
 // element-wise form: each element writes its own slot of each of its 6 sides).
@@ -262,16 +290,18 @@ __global__ void __launch_bounds__(Dm<N>::block)
 }
 
 // ---------------------------------------------------------------------------------------------------------
-// k_lift: lifted gradients and their face traces (ProlongToFaceLifting lifting_br1.t90:152-156).
+// k_lift: BR1 gradients and, on the element's own face nodes, the normal viscous flux
+// F_v(U, grad U).n_master (EvalNormalDiffFlux3D flux.f90:779-847 as ViscousFlux riemann.f90:638-700 applies it)
+// written to the element's slot of the viscous-flux traces.
 // ---------------------------------------------------------------------------------------------------------
 template <int N, typename T, typename TT>
 __global__ void __launch_bounds__(Dm<N>::block)
-    k_lift(const double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, TT* __restrict__ gtr,
+    k_lift(const double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, TT* __restrict__ vtr,
            const T* __restrict__ mats, GasD gd, T LHm, T LHp)
 {
     using D = Dm<N>;
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
-    constexpr int lds_a = 4 * n3 + 6 * 7 * n2, lds_b = 6 * 12 * n2;
+    constexpr int lds_a = 4 * n3 + 6 * 7 * n2, lds_b = 6 * 4 * n2;
     __shared__ T s_buf[lds_a > lds_b ? lds_a : lds_b];
     __shared__ Mats<N, T> M;
     const int e = blockIdx.x, tid = threadIdx.x;
@@ -291,17 +321,10 @@ __global__ void __launch_bounds__(Dm<N>::block)
     T g[12];
     lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_lift, s_lflux, gas, LHm, LHp, g);
     __syncthreads();
-    T* s_out = s_buf;  // [6][12][n2]
+    T* s_out = s_buf;  // [6][4][n2]
     if (tid < n3) {
         int i = tid % n, j = (tid / n) % n, k = tid / n2;
-#pragma unroll
-        for (int ls = 1; ls <= 6; ls++) {
-            int ab;
-            if (!on_face<N>(ls, i, j, k, ab)) continue;
-            T* o = s_out + (ls - 1) * 12 * n2 + ab;
-#pragma unroll
-            for (int q = 0; q < 12; q++) o[q * n2] = g[q];
-        }
+        store_face_viscous<N, T>(e, i, j, k, mesh, pr, g, gas, s_out);
     }
     __syncthreads();
     for (int t = tid; t < D::nf; t += D::block) {
@@ -310,10 +333,10 @@ __global__ void __launch_bounds__(Dm<N>::block)
         long side = c >> 3;
         int flip = c & 7;
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
-        TT* o = gtr + ((side * 2 + (flip ? 1 : 0)) * 12) * n2 + pq;
-        const T* s = s_out + (ls - 1) * 12 * n2 + ab;
+        TT* o = vtr + ((side * 2 + (flip ? 1 : 0)) * 4) * n2 + pq;
+        const T* s = s_out + (ls - 1) * 4 * n2 + ab;
 #pragma unroll
-        for (int q = 0; q < 12; q++) o[q * n2] = TT(s[q * n2]);
+        for (int q = 0; q < 4; q++) o[q * n2] = TT(s[q * n2]);
     }
 }
 
@@ -358,53 +381,55 @@ __global__ void __launch_bounds__(Dm<N>::block)
 //   else : VolInt_weakForm with EvalTransformedEulerDiffFlux3D (src/dg/volint.f90:129-188)
 //   FillFlux: Riemann + ViscousFlux + SurfElem (src/dg/fillflux.f90:45-172), SurfInt_Device (surfint.t90:352-586)
 //   Ut = -sJ*Ut (src/dg/dg.f90:413,423)
-// LDS: region W (work: lifting, volume fluxes, split-form partner data, face results, new traces) and region G
-// (own gradients on the six faces, written after the lifting and read by the face-flux tasks), so that the
-// gradient registers die before the volume phases.
+// LDS: region W (work: lifting, volume fluxes, split-form partner data, face results, new traces) and region V
+// (the own normal viscous fluxes on the six faces, formed right after the lifting and read by the face-flux
+// tasks), so that the gradient registers die before the volume phases.
 // ---------------------------------------------------------------------------------------------------------
 template <int N, typename T, typename TT, bool SPLIT, int RIEMANN>
-__device__ __forceinline__ void face_flux(const T* tro, const T* Go, const T* trn, const T* Gn,
+__device__ __forceinline__ void face_flux(const T* tro, const T* fvo, const T* trn, const T* fvn,
                                           const double* __restrict__ fg, int n2, int flip, const Gas<T>& gas, T* F)
 {
     T Uo[5], Un[5], Po[6], Pn[6];
     trace_state(tro, gas, Uo, Po);
     trace_state(trn, gas, Un, Pn);
     T nv[3] = {T(fg[0]), T(fg[n2]), T(fg[2 * n2])};
-    // ViscousFlux (riemann.f90:638-700): 0.5*(F_v(master) + F_v(slave)).n, formed before the inviscid part
-    T fvm[5], fvs[5];
-    normal_diff_flux(flip ? Pn : Po, flip ? Gn : Go, nv, gas, fvm);
-    normal_diff_flux(flip ? Po : Pn, flip ? Go : Gn, nv, gas, fvs);
     T t1[3] = {T(fg[3 * n2]), T(fg[4 * n2]), T(fg[5 * n2])};
     T t2[3] = {T(fg[6 * n2]), T(fg[7 * n2]), T(fg[8 * n2])};
     const T *UL = flip ? Un : Uo, *UR = flip ? Uo : Un, *PL = flip ? Pn : Po, *PR = flip ? Po : Pn;
     riemann_point<T, SPLIT>(RIEMANN, UL, UR, PL, PR, nv, t1, t2, gas, F);
+    // ViscousFlux (riemann.f90:638-700): 0.5*(F_v(master) + F_v(slave)).n
+    const T* fvm = flip ? fvn : fvo;
+    const T* fvs = flip ? fvo : fvn;
     T se = T(fg[9 * n2]);
+    F[0] = F[0] * se;
 #pragma unroll
-    for (int v = 0; v < 5; v++) F[v] = ((F[v] + T(0.5) * fvm[v]) + T(0.5) * fvs[v]) * se;
+    for (int v = 1; v < 5; v++) F[v] = ((F[v] + T(0.5) * fvm[v - 1]) + T(0.5) * fvs[v - 1]) * se;
 }
 
-// minimum waves per SIMD for k_rhs: 4 lets two 512-thread workgroups share a CU at N=7 (the register
-// allocation is capped at 128); the heavier entropy-fixed Roe solver keeps the compiler's choice.
-template <int RIEMANN>
+// minimum waves per SIMD for k_rhs: 4 lets two 512-thread workgroups share a CU at N=7 in FP64 (registers capped
+// at 128), 6 lets three share it in the mixed form (capped at 80, which its LDS share allows); the heavier
+// entropy-fixed Roe solver and the small blocks of N<5 keep the compiler's choice.
+template <int N, int RIEMANN, typename T>
 struct RhsWaves {
-    static constexpr int value = RIEMANN == RIEMANN_ROEEF ? 1 : 4;
+    static constexpr int value =
+        (RIEMANN == RIEMANN_ROEEF || Dm<N>::block < 256) ? 1 : (sizeof(T) == 4 ? 6 : 4);
 };
 
 template <int N, typename T, typename TT, bool SPLIT, int RIEMANN>
-__global__ void __launch_bounds__(Dm<N>::block, RhsWaves<RIEMANN>::value)
-    k_rhs(double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, const TT* __restrict__ gtr,
+__global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value))
+    k_rhs(double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, const TT* __restrict__ vtr,
           TT* __restrict__ utr_next, double* __restrict__ Ut, double* __restrict__ Kreg, const T* __restrict__ mats,
           GasD gd, T LHm, T LHp, Stage st)
 {
     using D = Dm<N>;
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
     constexpr int w_a = 4 * n3 + 6 * 7 * n2;          // lifting
-    constexpr int w_c = SPLIT ? 9 * n3 : 5 * n3;      // volume fluxes / split-form partner data
+    constexpr int w_c = SPLIT ? 13 * n3 : 5 * n3;     // volume fluxes / split-form partner data
     constexpr int w_d = 6 * 5 * n2;                   // face results, new traces
     constexpr int w_ac = w_a > w_c ? w_a : w_c;
     constexpr int lds_w = w_ac > w_d ? w_ac : w_d;
     __shared__ T s_w[lds_w];
-    __shared__ T s_g[6 * 12 * n2];
+    __shared__ T s_fv[6 * 4 * n2];
     __shared__ Mats<N, T> M;
     const int e = blockIdx.x, tid = threadIdx.x;
     const bool active = tid < n3;
@@ -426,50 +451,46 @@ __global__ void __launch_bounds__(Dm<N>::block, RhsWaves<RIEMANN>::value)
         lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3, gas, LHm, LHp, g);
         if (active) {
             stress(pr, g, gas, s);
-#pragma unroll
-            for (int ls = 1; ls <= 6; ls++) {
-                int ab;
-                if (!on_face<N>(ls, i, j, k, ab)) continue;
-                T* o = s_g + (ls - 1) * 12 * n2 + ab;
-#pragma unroll
-                for (int q = 0; q < 12; q++) o[q * n2] = g[q];
-            }
+            store_face_viscous<N, T>(e, i, j, k, mesh, pr, g, gas, s_fv);
         }
     }
     __syncthreads();
 
-    // ---- viscous (split) or full (weak) transformed fluxes, weak-form D_Hat_T contraction ----
     T ut[5] = {0, 0, 0, 0, 0};
     const double* ge = mesh.geo + (long)e * 10 * n3 + tid;
-    T* s_f = s_w;  // [5][n3]
+    if (!SPLIT) {
+        // ---- full transformed fluxes, weak-form D_Hat_T contraction (VolInt_weakForm) ----
+        T* s_f = s_w;  // [5][n3]
 #pragma unroll 1
-    for (int d = 0; d < 3; d++) {
-        if (active) {
-            T m[3] = {T(ge[(3 * d) * n3]), T(ge[(3 * d + 1) * n3]), T(ge[(3 * d + 2) * n3])};
-            T f[5];
-            transformed_flux<T, !SPLIT>(uc, pr, s, m, gas.hshift, f);
+        for (int d = 0; d < 3; d++) {
+            if (active) {
+                T m[3] = {T(ge[(3 * d) * n3]), T(ge[(3 * d + 1) * n3]), T(ge[(3 * d + 2) * n3])};
+                T f[5];
+                transformed_flux<T, true>(uc, pr, s, m, gas.hshift, f);
 #pragma unroll
-            for (int v = (SPLIT ? 1 : 0); v < 5; v++) s_f[v * n3 + tid] = f[v];
-        }
-        __syncthreads();
-        if (active) {
-            int idx = d == 0 ? i : (d == 1 ? j : k);
-            int stride = d == 0 ? 1 : (d == 1 ? n : n2);
-            int base = tid - idx * stride;
-#pragma unroll GALX_UNROLL_L
-            for (int l = 0; l <= N; l++) {
-                T dh = M.DhT[l + n * idx];
-#pragma unroll
-                for (int v = (SPLIT ? 1 : 0); v < 5; v++) ut[v] = ut[v] + dh * s_f[v * n3 + base + l * stride];
+                for (int v = 0; v < 5; v++) s_f[v * n3 + tid] = f[v];
             }
+            __syncthreads();
+            if (active) {
+                int idx = d == 0 ? i : (d == 1 ? j : k);
+                int stride = d == 0 ? 1 : (d == 1 ? n : n2);
+                int base = tid - idx * stride;
+#pragma unroll GALX_UNROLL_L
+                for (int l = 0; l <= N; l++) {
+                    T dh = M.DhT[l + n * idx];
+#pragma unroll
+                    for (int v = 0; v < 5; v++) ut[v] = ut[v] + dh * s_f[v * n3 + base + l * stride];
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-    }
-
-    // ---- split-form flux differencing (VolInt_splitForm_Kernel) ----
-    if (SPLIT) {
+    } else {
+        // ---- per direction: viscous contravariant flux with the weak-form D_Hat_T contraction
+        // (VolInt_weakForm_Visc) and the split-form flux differencing (VolInt_splitForm_Kernel); the direction's
+        // metric is read once for both ----
         T* s_v = s_w;           // [6][n3] rho,u,v,w,p,H
         T* s_m = s_w + 6 * n3;  // [3][n3] metric of the current direction
+        T* s_f = s_w + 9 * n3;  // [4][n3] viscous contravariant flux (momentum, energy) of the current direction
         if (active) {
             s_v[tid] = pr[0];
             s_v[n3 + tid] = pr[1];
@@ -489,12 +510,22 @@ __global__ void __launch_bounds__(Dm<N>::block, RhsWaves<RIEMANN>::value)
                 s_m[tid] = mme[0];
                 s_m[n3 + tid] = mme[1];
                 s_m[2 * n3 + tid] = mme[2];
+                T f[5];
+                transformed_flux<T, false>(uc, pr, s, mme, gas.hshift, f);
+#pragma unroll
+                for (int v = 1; v < 5; v++) s_f[(v - 1) * n3 + tid] = f[v];
             }
             __syncthreads();
             if (active) {
                 int idx = d == 0 ? i : (d == 1 ? j : k);
                 int stride = d == 0 ? 1 : (d == 1 ? n : n2);
                 int base = tid - idx * stride;
+#pragma unroll GALX_UNROLL_L
+                for (int l = 0; l <= N; l++) {
+                    T dh = M.DhT[l + n * idx];
+#pragma unroll
+                    for (int v = 1; v < 5; v++) ut[v] = ut[v] + dh * s_f[(v - 1) * n3 + base + l * stride];
+                }
 #pragma unroll GALX_UNROLL_SPLIT
                 for (int l = 0; l <= N; l++) {
                     if (l == idx) continue;  // DVolSurf diagonal vanishes for Gauss-Lobatto (src/dg/dg.f90:229-237)
@@ -524,7 +555,7 @@ __global__ void __launch_bounds__(Dm<N>::block, RhsWaves<RIEMANN>::value)
         long side = c >> 3;
         int flip = c & 7;
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
-        T Uo[5], Go[12], Un[5], Gn[12];
+        T Uo[5], fvo[4], Un[5], fvn[4];
         {
             const double* uo = U + (long)e * 5 * n3 + node;
             double c[5], tr[5];
@@ -534,18 +565,18 @@ __global__ void __launch_bounds__(Dm<N>::block, RhsWaves<RIEMANN>::value)
 #pragma unroll
             for (int v = 0; v < 5; v++) Uo[v] = T(TT(tr[v]));  // exactly the record the neighbour reads
         }
-        const T* so = s_g + (ls - 1) * 12 * n2 + ab;
+        const T* so = s_fv + (ls - 1) * 4 * n2 + ab;
 #pragma unroll
-        for (int q = 0; q < 12; q++) Go[q] = so[q * n2];
+        for (int q = 0; q < 4; q++) fvo[q] = so[q * n2];
         const long nslot = side * 2 + (flip ? 0 : 1);
         const TT* un = utr + nslot * 5 * n2 + pq;
-        const TT* gn = gtr + nslot * 12 * n2 + pq;
+        const TT* vn = vtr + nslot * 4 * n2 + pq;
 #pragma unroll
         for (int v = 0; v < 5; v++) Un[v] = T(un[v * n2]);
 #pragma unroll
-        for (int q = 0; q < 12; q++) Gn[q] = T(gn[q * n2]);
+        for (int q = 0; q < 4; q++) fvn[q] = T(vn[q * n2]);
         T F[5];
-        face_flux<N, T, TT, SPLIT, RIEMANN>(Uo, Go, Un, Gn, mesh.fgeo + side * 10 * n2 + pq, n2, flip, gas, F);
+        face_flux<N, T, TT, SPLIT, RIEMANN>(Uo, fvo, Un, fvn, mesh.fgeo + side * 10 * n2 + pq, n2, flip, gas, F);
         T sg = flip ? T(-1) : T(1);
 #pragma unroll
         for (int v = 0; v < 5; v++) s_res[(ls - 1) * 5 * n2 + v * n2 + ab] = sg * F[v];

@@ -59,6 +59,17 @@ INTERFACE
     REAL(C_DOUBLE),VALUE :: t, dt
     INTEGER(C_INT),VALUE :: nSteps
   END FUNCTION
+  INTEGER(C_INT) FUNCTION galx_lserk_stage(ctx,t,dt,iStage) BIND(C,NAME='galx_lserk_stage')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR),VALUE    :: ctx
+    REAL(C_DOUBLE),VALUE :: t, dt
+    INTEGER(C_INT),VALUE :: iStage
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION galx_get_gradients(ctx,gradUx,gradUy,gradUz) BIND(C,NAME='galx_get_gradients')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR),VALUE          :: ctx
+    REAL(C_DOUBLE),INTENT(OUT) :: gradUx(*), gradUy(*), gradUz(*)
+  END FUNCTION
   INTEGER(C_INT) FUNCTION galx_calc_dt(ctx,dt,errType) BIND(C,NAME='galx_calc_dt')
     IMPORT :: C_INT, C_PTR, C_DOUBLE
     TYPE(C_PTR),VALUE           :: ctx
