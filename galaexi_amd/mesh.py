@@ -164,6 +164,138 @@ def _surf_metrics_from_ja(normal_dir, tang_dir, normal_sign, Ja):
     return nv, t1, t2, se
 
 
+class _Geometry:
+    """Volume and surface metrics from the equidistant NGeo element nodes (metrics.f90:171-793): the nodes are
+    interpolated to Chebyshev-Lobatto points, the Jacobian determinant is evaluated on GL points of degree
+    3*NGeo and projected to N, the metric terms use the curl form on the CL points of degree N and are
+    interpolated to the GL solution points."""
+
+    def __init__(self, N, NGeo):
+        self.N, self.NGeo = N, NGeo
+        self.xEQ, _w, _wb = basis.nodes_and_weights(NGeo, basis.VISU)
+        xCLg, _w, _wb = basis.nodes_and_weights(NGeo, basis.CHEBYSHEV_LOBATTO)
+        xCLn, _w, _wb = basis.nodes_and_weights(N, basis.CHEBYSHEV_LOBATTO)
+        self.xGL, _w, _wb = basis.nodes_and_weights(N, basis.GAUSS_LOBATTO)
+        NGeoRef = 3 * NGeo
+        self.V_EQ_CL = basis.vandermonde(NGeo, basis.VISU, NGeo, basis.CHEBYSHEV_LOBATTO)
+        self.V_CLg_CLn = basis.vandermonde(NGeo, basis.CHEBYSHEV_LOBATTO, N, basis.CHEBYSHEV_LOBATTO)
+        self.V_CLg_Ref = basis.vandermonde(NGeo, basis.CHEBYSHEV_LOBATTO, NGeoRef, basis.GAUSS_LOBATTO)
+        self.V_Ref_N = basis.vandermonde(NGeoRef, basis.GAUSS_LOBATTO, N, basis.GAUSS_LOBATTO, modal=True)
+        self.V_CLn_N = basis.vandermonde(N, basis.CHEBYSHEV_LOBATTO, N, basis.GAUSS_LOBATTO)
+        self.V_CLg_N = basis.vandermonde(NGeo, basis.CHEBYSHEV_LOBATTO, N, basis.GAUSS_LOBATTO)
+        self.DCLg = basis.derivative_matrix(xCLg)
+        self.DCLn = basis.derivative_matrix(xCLn)
+        self.s2v2 = build_s2v2(N)
+
+    def volume(self, X):
+        """X(e, comp, i, j, k) on the equidistant NGeo nodes -> Ja(e, dir, comp, i, j, k), sJ(e, i, j, k) on the GL
+        points, JaCL(e, dir, comp, i, j, k) on the CL points of degree N, xGP(e, comp, i, j, k)."""
+        N, NGeo = self.N, self.NGeo
+        DCLg, DCLn = self.DCLg, self.DCLn
+        XCLg = _tensor3(self.V_EQ_CL, self.V_EQ_CL, self.V_EQ_CL, X)
+        XCLn = _tensor3(self.V_CLg_CLn, self.V_CLg_CLn, self.V_CLg_CLn, XCLg)
+        dXg = np.stack([_along_i(DCLg, XCLg), _along_j(DCLg, XCLg), _along_k(DCLg, XCLg)], axis=1)  # (e,dir,comp,..)
+        a = _tensor3(self.V_CLg_Ref, self.V_CLg_Ref, self.V_CLg_Ref, dXg)
+        det = (a[:, 0, 0] * (a[:, 1, 1] * a[:, 2, 2] - a[:, 2, 1] * a[:, 1, 2]) +
+               a[:, 1, 0] * (a[:, 2, 1] * a[:, 0, 2] - a[:, 0, 1] * a[:, 2, 2]) +
+               a[:, 2, 0] * (a[:, 0, 1] * a[:, 1, 2] - a[:, 1, 1] * a[:, 0, 2]))
+        detN = _tensor3(self.V_Ref_N, self.V_Ref_N, self.V_Ref_N, det)
+        sJe = 1.0 / detN
+        if N >= NGeo:
+            dXn = _tensor3(self.V_CLg_CLn, self.V_CLg_CLn, self.V_CLg_CLn, dXg)
+        else:
+            dXn = np.stack([_along_i(DCLn, XCLn), _along_j(DCLn, XCLn), _along_k(DCLn, XCLn)], axis=1)
+        # curl form (metrics.f90:450-484): Rc(e, dir, comp)
+        Xc = XCLn[:, None]
+        Rc = np.empty_like(dXn)
+        Rc[:, :, 0] = 0.5 * (Xc[:, :, 2] * dXn[:, :, 1] - Xc[:, :, 1] * dXn[:, :, 2])
+        Rc[:, :, 1] = 0.5 * (Xc[:, :, 0] * dXn[:, :, 2] - Xc[:, :, 2] * dXn[:, :, 0])
+        Rc[:, :, 2] = 0.5 * (Xc[:, :, 1] * dXn[:, :, 0] - Xc[:, :, 0] * dXn[:, :, 1])
+        JaCL = np.empty_like(dXn)
+        JaCL[:, 0] = -_along_j(DCLn, Rc[:, 2]) + _along_k(DCLn, Rc[:, 1])
+        JaCL[:, 1] = -_along_k(DCLn, Rc[:, 0]) + _along_i(DCLn, Rc[:, 2])
+        JaCL[:, 2] = -_along_i(DCLn, Rc[:, 1]) + _along_j(DCLn, Rc[:, 0])
+        Ja = _tensor3(self.V_CLn_N, self.V_CLn_N, self.V_CLn_N, JaCL)
+        xg = _tensor3(self.V_CLg_N, self.V_CLg_N, self.V_CLg_N, XCLg)
+        return Ja, sJe, JaCL, xg
+
+    def face(self, JaCL_m, ls):
+        """Ja on the master faces of local side ls in side (p,q) order (metrics.f90:700-740).
+        JaCL_m(m, dir, comp, i, j, k) -> (dir, comp, p, q, m)"""
+        N = self.N
+        if ls == XI_MINUS:
+            F = JaCL_m[:, :, :, 0, :, :]
+        elif ls == XI_PLUS:
+            F = JaCL_m[:, :, :, N, :, :]
+        elif ls == ETA_MINUS:
+            F = JaCL_m[:, :, :, :, 0, :]
+        elif ls == ETA_PLUS:
+            F = JaCL_m[:, :, :, :, N, :]
+        elif ls == ZETA_MINUS:
+            F = JaCL_m[:, :, :, :, :, 0]
+        else:
+            F = JaCL_m[:, :, :, :, :, N]
+        F = np.matmul(np.matmul(self.V_CLn_N, F), self.V_CLn_N.T)  # (m, dir, comp, a, b)
+        A, B = self.s2v2[0, :, :, 0, ls - 1], self.s2v2[1, :, :, 0, ls - 1]
+        return np.moveaxis(F[..., A, B], 0, -1)
+
+
+class _GeometryArrays:
+    """The geometry arrays in the reference layouts (Fortran order)."""
+
+    def __init__(self, N, nE, nS):
+        n = N + 1
+        self.mf = np.zeros((3, n, n, n, nE), order="F")
+        self.mg = np.zeros((3, n, n, n, nE), order="F")
+        self.mh = np.zeros((3, n, n, n, nE), order="F")
+        self.sJ = np.zeros((n, n, n, nE), order="F")
+        self.xGP = np.zeros((3, n, n, n, nE), order="F")
+        self.nv = np.zeros((3, n, n, nS), order="F")
+        self.t1 = np.zeros((3, n, n, nS), order="F")
+        self.t2 = np.zeros((3, n, n, nS), order="F")
+        self.se = np.zeros((n, n, nS), order="F")
+
+
+def fill_geometry(geo, nE, s2e, nodes, chunk=256, threads=None):
+    """Volume metrics of every element and surface metrics of every side, the latter from the side's master
+    element (CalcSurfMetrics metrics.f90:611-740 skips flip /= 0).  nodes(elem_ids) returns the equidistant
+    NGeo nodes X(e, comp, i, j, k) of a chunk of elements; chunks run on `threads` worker threads."""
+    nS = s2e.shape[1]
+    G = _GeometryArrays(geo.N, nE, nS)
+    master_elem = s2e[0, :] - 1
+    ls_of_side = s2e[2, :]
+
+    def fill(c0):
+        ids = np.arange(c0, min(nE, c0 + chunk))
+        Ja, sJc, JaCL, xg = geo.volume(nodes(ids))
+        G.mf[..., ids] = np.moveaxis(Ja[:, 0], 0, -1)
+        G.mg[..., ids] = np.moveaxis(Ja[:, 1], 0, -1)
+        G.mh[..., ids] = np.moveaxis(Ja[:, 2], 0, -1)
+        G.sJ[..., ids] = np.moveaxis(sJc, 0, -1)
+        G.xGP[..., ids] = np.moveaxis(xg, 0, -1)
+        in_chunk = (master_elem >= c0) & (master_elem < c0 + len(ids))
+        for ls in range(1, 7):
+            sides = np.nonzero(in_chunk & (ls_of_side == ls))[0]
+            if len(sides) == 0:
+                continue
+            JaF = geo.face(JaCL[master_elem[sides] - c0], ls)
+            a, b, c, d = _surf_metrics_from_ja(NORMAL_DIRS[ls - 1], TANG_DIRS[ls - 1], NORMAL_SIGNS[ls - 1], JaF)
+            G.nv[:, :, :, sides] = a
+            G.t1[:, :, :, sides] = b
+            G.t2[:, :, :, sides] = c
+            G.se[:, :, sides] = d
+
+    # chunks are independent and write disjoint slices; numpy releases the GIL in matmul and ufuncs
+    workers = max(1, min(threads or os.cpu_count() or 1, 16, (nE + chunk - 1) // chunk))
+    if workers == 1:
+        for c0 in range(0, nE, chunk):
+            fill(c0)
+    else:
+        with ThreadPoolExecutor(workers) as ex:
+            list(ex.map(fill, range(0, nE, chunk)))
+    return G
+
+
 def box_mesh(nelems, N, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), deform_amp=0.0, deform_phase=(0.0, 0.0, 0.0),
              NGeo=None, rotations=None, seed=None, chunk=256, threads=None):
     """Periodic box of nelems=(nx,ny,nz) hexahedra with polynomial degree N.
@@ -236,26 +368,13 @@ def box_mesh(nelems, N, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), deform_amp=0.0, 
     s2e = np.asfortranarray(np.array(s2e_list, dtype=np.int32).T)
 
     # ---- geometry (metrics.f90:171-610) ----
-    xEQ, _w, _wb = basis.nodes_and_weights(NGeo, basis.VISU)
-    xCLg, _w, _wb = basis.nodes_and_weights(NGeo, basis.CHEBYSHEV_LOBATTO)
-    xCLn, _w, _wb = basis.nodes_and_weights(N, basis.CHEBYSHEV_LOBATTO)
-    xGL, _w, _wb = basis.nodes_and_weights(N, basis.GAUSS_LOBATTO)
-    NGeoRef = 3 * NGeo
-    V_EQ_CL = basis.vandermonde(NGeo, basis.VISU, NGeo, basis.CHEBYSHEV_LOBATTO)
-    V_CLg_CLn = basis.vandermonde(NGeo, basis.CHEBYSHEV_LOBATTO, N, basis.CHEBYSHEV_LOBATTO)
-    V_CLg_Ref = basis.vandermonde(NGeo, basis.CHEBYSHEV_LOBATTO, NGeoRef, basis.GAUSS_LOBATTO)
-    V_Ref_N = basis.vandermonde(NGeoRef, basis.GAUSS_LOBATTO, N, basis.GAUSS_LOBATTO, modal=True)
-    V_CLn_N = basis.vandermonde(N, basis.CHEBYSHEV_LOBATTO, N, basis.GAUSS_LOBATTO)
-    V_CLg_N = basis.vandermonde(NGeo, basis.CHEBYSHEV_LOBATTO, N, basis.GAUSS_LOBATTO)
-    DCLg = basis.derivative_matrix(xCLg)
-    DCLn = basis.derivative_matrix(xCLn)
+    geo = _Geometry(N, NGeo)
     h = (hi - lo) / ncell
-
     uniform = deform_amp == 0.0 and np.all(rot == 0)
+    xEQ = geo.xEQ
 
-    def geometry(elem_ids, centre_override=None):
-        """Metrics of a chunk of elements, element axis first: Ja(e, dir, comp, i, j, k), sJ(e, i, j, k),
-        JaCL(e, dir, comp, i, j, k) on the CL nodes of degree N, xGP(e, comp, i, j, k)."""
+    def nodes(elem_ids, centre_override=None):
+        """Equidistant NGeo nodes X(e, comp, i, j, k) of a chunk of elements."""
         ne = len(elem_ids)
         xi = np.stack(np.meshgrid(xEQ, xEQ, xEQ, indexing="ij"), axis=0)  # (3, i, j, k)
         R = np.stack(_ROTS)[rot[elem_ids]].astype(np.float64)  # (ne,3,3)
@@ -271,113 +390,34 @@ def box_mesh(nelems, N, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), deform_amp=0.0, 
             ph = np.asarray(deform_phase, dtype=np.float64)[None, :, None, None, None]
             disp = deform_amp * np.prod(np.sin(np.pi * (xh + ph)), axis=1)
             X = X + 0.5 * span * disp[:, None]
-        XCLg = _tensor3(V_EQ_CL, V_EQ_CL, V_EQ_CL, X)
-        XCLn = _tensor3(V_CLg_CLn, V_CLg_CLn, V_CLg_CLn, XCLg)
-        dXg = np.stack([_along_i(DCLg, XCLg), _along_j(DCLg, XCLg), _along_k(DCLg, XCLg)], axis=1)  # (e,dir,comp,..)
-        a = _tensor3(V_CLg_Ref, V_CLg_Ref, V_CLg_Ref, dXg)
-        det = (a[:, 0, 0] * (a[:, 1, 1] * a[:, 2, 2] - a[:, 2, 1] * a[:, 1, 2]) +
-               a[:, 1, 0] * (a[:, 2, 1] * a[:, 0, 2] - a[:, 0, 1] * a[:, 2, 2]) +
-               a[:, 2, 0] * (a[:, 0, 1] * a[:, 1, 2] - a[:, 1, 1] * a[:, 0, 2]))
-        detN = _tensor3(V_Ref_N, V_Ref_N, V_Ref_N, det)
-        sJe = 1.0 / detN
-        if N >= NGeo:
-            dXn = _tensor3(V_CLg_CLn, V_CLg_CLn, V_CLg_CLn, dXg)
-        else:
-            dXn = np.stack([_along_i(DCLn, XCLn), _along_j(DCLn, XCLn), _along_k(DCLn, XCLn)], axis=1)
-        # curl form (metrics.f90:450-484): Rc(e, dir, comp)
-        Xc = XCLn[:, None]
-        Rc = np.empty_like(dXn)
-        Rc[:, :, 0] = 0.5 * (Xc[:, :, 2] * dXn[:, :, 1] - Xc[:, :, 1] * dXn[:, :, 2])
-        Rc[:, :, 1] = 0.5 * (Xc[:, :, 0] * dXn[:, :, 2] - Xc[:, :, 2] * dXn[:, :, 0])
-        Rc[:, :, 2] = 0.5 * (Xc[:, :, 1] * dXn[:, :, 0] - Xc[:, :, 0] * dXn[:, :, 1])
-        JaCL = np.empty_like(dXn)
-        JaCL[:, 0] = -_along_j(DCLn, Rc[:, 2]) + _along_k(DCLn, Rc[:, 1])
-        JaCL[:, 1] = -_along_k(DCLn, Rc[:, 0]) + _along_i(DCLn, Rc[:, 2])
-        JaCL[:, 2] = -_along_i(DCLn, Rc[:, 1]) + _along_j(DCLn, Rc[:, 0])
-        Ja = _tensor3(V_CLn_N, V_CLn_N, V_CLn_N, JaCL)
-        xg = _tensor3(V_CLg_N, V_CLg_N, V_CLg_N, XCLg)
-        return Ja, sJe, JaCL, xg
+        return X
 
-    mf = np.zeros((3, n, n, n, nE), order="F")
-    mg = np.zeros((3, n, n, n, nE), order="F")
-    mh = np.zeros((3, n, n, n, nE), order="F")
-    sJ = np.zeros((n, n, n, nE), order="F")
-    xGP = np.zeros((3, n, n, n, nE), order="F")
-    nv = np.zeros((3, n, n, nS), order="F")
-    t1 = np.zeros((3, n, n, nS), order="F")
-    t2 = np.zeros((3, n, n, nS), order="F")
-    se = np.zeros((n, n, nS), order="F")
-
-    def face_metrics(JaCL_m, ls):
-        """Ja on the master faces of local side ls in side (p,q) order (metrics.f90:700-740).
-        JaCL_m(m, dir, comp, i, j, k) -> (dir, comp, p, q, m)"""
-        if ls == XI_MINUS:
-            F = JaCL_m[:, :, :, 0, :, :]
-        elif ls == XI_PLUS:
-            F = JaCL_m[:, :, :, N, :, :]
-        elif ls == ETA_MINUS:
-            F = JaCL_m[:, :, :, :, 0, :]
-        elif ls == ETA_PLUS:
-            F = JaCL_m[:, :, :, :, N, :]
-        elif ls == ZETA_MINUS:
-            F = JaCL_m[:, :, :, :, :, 0]
-        else:
-            F = JaCL_m[:, :, :, :, :, N]
-        F = np.matmul(np.matmul(V_CLn_N, F), V_CLn_N.T)  # (m, dir, comp, a, b)
-        A, B = s2v2[0, :, :, 0, ls - 1], s2v2[1, :, :, 0, ls - 1]
-        return np.moveaxis(F[..., A, B], 0, -1)
-
-    ls_of_side = s2e[2, :]
     if uniform:
-        Ja1, sJ1, JaCL1, _x = geometry(np.array([0]), centre_override=np.zeros(3))
-        mf[:] = Ja1[0, 0][..., None]
-        mg[:] = Ja1[0, 1][..., None]
-        mh[:] = Ja1[0, 2][..., None]
-        sJ[:] = sJ1[0][..., None]
+        # every element has the metrics of element 0: compute one and broadcast
+        G = _GeometryArrays(N, nE, nS)
+        Ja1, sJ1, JaCL1, _x = geo.volume(nodes(np.array([0]), centre_override=np.zeros(3)))
+        G.mf[:] = Ja1[0, 0][..., None]
+        G.mg[:] = Ja1[0, 1][..., None]
+        G.mh[:] = Ja1[0, 2][..., None]
+        G.sJ[:] = sJ1[0][..., None]
+        xGL = geo.xGL
         xi3 = np.stack(np.meshgrid(xGL, xGL, xGL, indexing="ij"), axis=0)
         centre = (lo[:, None] + (cells.T + 0.5) * h[:, None])
-        xGP[:] = centre[:, None, None, None, :] + 0.5 * h[:, None, None, None, None] * xi3[..., None]
+        G.xGP[:] = centre[:, None, None, None, :] + 0.5 * h[:, None, None, None, None] * xi3[..., None]
+        ls_of_side = s2e[2, :]
         for ls in range(1, 7):
             ids = np.nonzero(ls_of_side == ls)[0]
             if len(ids) == 0:
                 continue
             a, b, c, d = _surf_metrics_from_ja(NORMAL_DIRS[ls - 1], TANG_DIRS[ls - 1], NORMAL_SIGNS[ls - 1],
-                                               face_metrics(JaCL1, ls))
-            nv[:, :, :, ids] = a
-            t1[:, :, :, ids] = b
-            t2[:, :, :, ids] = c
-            se[:, :, ids] = d
+                                               geo.face(JaCL1, ls))
+            G.nv[:, :, :, ids] = a
+            G.t1[:, :, :, ids] = b
+            G.t2[:, :, :, ids] = c
+            G.se[:, :, ids] = d
     else:
-        master_elem = s2e[0, :] - 1
-
-        def fill(c0):
-            ids = np.arange(c0, min(nE, c0 + chunk))
-            Ja, sJc, JaCL, xg = geometry(ids)
-            mf[..., ids] = np.moveaxis(Ja[:, 0], 0, -1)
-            mg[..., ids] = np.moveaxis(Ja[:, 1], 0, -1)
-            mh[..., ids] = np.moveaxis(Ja[:, 2], 0, -1)
-            sJ[..., ids] = np.moveaxis(sJc, 0, -1)
-            xGP[..., ids] = np.moveaxis(xg, 0, -1)
-            in_chunk = (master_elem >= c0) & (master_elem < c0 + len(ids))
-            for ls in range(1, 7):
-                sides = np.nonzero(in_chunk & (ls_of_side == ls))[0]
-                if len(sides) == 0:
-                    continue
-                JaF = face_metrics(JaCL[master_elem[sides] - c0], ls)
-                a, b, c, d = _surf_metrics_from_ja(NORMAL_DIRS[ls - 1], TANG_DIRS[ls - 1], NORMAL_SIGNS[ls - 1], JaF)
-                nv[:, :, :, sides] = a
-                t1[:, :, :, sides] = b
-                t2[:, :, :, sides] = c
-                se[:, :, sides] = d
-
-        # chunks are independent and write disjoint slices; numpy releases the GIL in matmul and ufuncs
-        workers = max(1, min(threads or os.cpu_count() or 1, 16, (nE + chunk - 1) // chunk))
-        if workers == 1:
-            for c0 in range(0, nE, chunk):
-                fill(c0)
-        else:
-            with ThreadPoolExecutor(workers) as ex:
-                list(ex.map(fill, range(0, nE, chunk)))
+        G = fill_geometry(geo, nE, s2e, nodes, chunk=chunk, threads=threads)
+    mf, mg, mh, sJ, xGP, nv, t1, t2, se = G.mf, G.mg, G.mh, G.sJ, G.xGP, G.nv, G.t1, G.t2, G.se
     return Mesh(N=N, nElems=nE, nSides=nS, ElemToSide=e2s, SideToElem=s2e, S2V2=s2v2,
                 Metrics_fTilde=mf, Metrics_gTilde=mg, Metrics_hTilde=mh, sJ=sJ, NormVec=nv, TangVec1=t1,
                 TangVec2=t2, SurfElem=se, Elem_xGP=xGP, lo=tuple(lo), hi=tuple(hi),

@@ -1,0 +1,33 @@
+"""Copy the reference's HDF5 mesh and state fixtures into tests/golden/ unchanged (data files of the reference's
+own regression checks, GPL-3.0 reference repository), and print their SHA-256.
+
+  CART_HEX_PERIODIC_008_mesh.h5        regressioncheck/checks/tgv/split/ (the TGV regression case's mesh:
+                                       8^3 periodic box [0,2pi]^3, NGeo=1, HOPR output)
+  cartbox3D_mesh.h5                    regressioncheck/checks/timedisc/freestream_3D/ (2^3 box with
+                                       non-periodic boundaries, BC type 2)
+  cartbox_State_0000000.000001000.h5   regressioncheck/checks/timedisc/freestream_3D/ (a DG_Solution state file,
+                                       N=3, written by the reference at t=1e-6 for that mesh)
+
+Usage: python tests/golden/copy_hdf5_fixtures.py [/path/to/reference]
+"""
+import hashlib
+import os
+import shutil
+import sys
+
+FILES = ("regressioncheck/checks/tgv/split/CART_HEX_PERIODIC_008_mesh.h5",
+         "regressioncheck/checks/timedisc/freestream_3D/cartbox3D_mesh.h5",
+         "regressioncheck/checks/timedisc/freestream_3D/cartbox_State_0000000.000001000.h5")
+
+
+def main(ref_root):
+    here = os.path.dirname(os.path.abspath(__file__))
+    for rel in FILES:
+        dst = os.path.join(here, os.path.basename(rel))
+        shutil.copyfile(os.path.join(ref_root, rel), dst)
+        with open(dst, "rb") as f:
+            print(os.path.basename(rel), hashlib.sha256(f.read()).hexdigest())
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
