@@ -115,21 +115,13 @@ def kinetic_energy_grad(W, U):
     return g
 
 
-def run_adjoint(ne, steps, every, fd_eps):
-    """J = int 1/2 rho |u|^2 dV after `steps` LSERK steps of the curved TGV case (FP64); dJ/dU0 by the
-    checkpointed discrete adjoint.  Cost ratio = wall of the reverse sweep (segment recompute + stage VJPs)
-    over the wall of the plain forward integration; a central finite difference along one seeded direction
-    checks the gradient."""
+def _adjoint_leg(m, U0, W, dt, steps, every, precision):
+    """Forward wall, checkpointed forward wall, reverse-sweep wall, dJ/dU0 and kernel timers of one context."""
     from galaexi_amd import dg
 
-    N = 7
-    m, U0 = tgv_case(ne, N, curved=True)
-    W = mass_weights(m)
-    op = dg.DGOperator(N, form="split", riemann="lf", precision="fp64", **TGV_GAS)
+    op = dg.DGOperator(m.N, form="split", riemann="lf", precision=precision, **TGV_GAS)
     op.set_mesh(m)
     op.set_state(U0)
-    dt, err = op.CalcTimeStep()
-    assert err == 0, err
     op.TimeStepByLSERKW2(0.0, dt, 2)  # warm-up
     op.set_state(U0)
     t0 = time.perf_counter()
@@ -144,11 +136,32 @@ def run_adjoint(ne, steps, every, fd_eps):
     op.timers(True)
     op.timers_reset()
     t0 = time.perf_counter()
-    g0 = op.lserk_adjoint(kinetic_energy_grad(W, UN))
+    g = op.lserk_adjoint(kinetic_energy_grad(W, UN))
     t_adj = time.perf_counter() - t0
     ker = {k: op.timer(k) for k in ("trace", "lift", "rhs", "adjoint")}
     op.timers(False)
-    mem = op.memory_bytes()
+    return op, dict(t_fwd=t_fwd, t_rec=t_rec, t_adj=t_adj, g=g, J=J, ker=ker, mem=op.memory_bytes())
+
+
+def run_adjoint(ne, steps, every, fd_eps, mixed_leg=True):
+    """J = int 1/2 rho |u|^2 dV after `steps` LSERK steps of the curved TGV case; dJ/dU0 by the checkpointed
+    discrete adjoint.  Cost ratio = wall of the reverse sweep (segment recompute + stage VJPs) over the wall of
+    the plain forward integration.  FP64 leg: a central finite difference along one seeded direction checks
+    the gradient.  Mixed leg: the forward (and the segment recompute) in the mixed form, the adjoint accumulated
+    in FP64 along that trajectory; its gradient is compared with the FP64 leg's and with the same FD value."""
+    N = 7
+    m, U0 = tgv_case(ne, N, curved=True)
+    W = mass_weights(m)
+    from galaexi_amd import dg
+
+    probe = dg.DGOperator(N, form="split", riemann="lf", precision="fp64", **TGV_GAS)
+    probe.set_mesh(m)
+    probe.set_state(U0)
+    dt, err = probe.CalcTimeStep()
+    probe.close()
+    assert err == 0, err
+    op, r = _adjoint_leg(m, U0, W, dt, steps, every, "fp64")
+    g0 = r["g"]
     rng = np.random.default_rng(400)
     s = np.abs(U0).reshape(5, -1).mean(axis=1).reshape(5, 1, 1, 1, 1)
     v = np.asfortranarray(0.1 * s * rng.standard_normal(U0.shape))
@@ -164,44 +177,70 @@ def run_adjoint(ne, steps, every, fd_eps):
         fd[f"{eps:g}"] = abs(cd[eps] - exact) / abs(exact)
     # Richardson extrapolation of the two largest steps (2e, e) removes the O(e^2) truncation term
     e2, e1 = sorted(cd)[-1], sorted(cd)[-2]
+    fd_best = cd[e1]
     if abs(e2 - 2 * e1) < 1e-12 * e2:
-        fd[f"richardson({e1:g},{e2:g})"] = abs((4 * cd[e1] - cd[e2]) / 3 - exact) / abs(exact)
+        fd_best = (4 * cd[e1] - cd[e2]) / 3
+        fd[f"richardson({e1:g},{e2:g})"] = abs(fd_best - exact) / abs(exact)
     op.close()
-    return {
+    ker = r["ker"]
+    out = {
         "workload": f"TGV Re1600 Ma0.1 [0,2pi]^3 {ne}^3 elems N={N}, curved (0.05 sine mapping), split-PI + LF, "
                     f"BR1, FP64; J = int 1/2 rho|u|^2 dV after {steps} LSERK steps, checkpoints every {every}",
-        "nDOF": m.nDOF, "dt": dt, "J": J,
-        "forward_s": t_fwd, "forward_with_checkpoints_s": t_rec, "adjoint_sweep_s": t_adj,
-        "adjoint_over_forward": t_adj / t_fwd,
-        "gradient_total_over_forward": (t_rec + t_adj) / t_fwd,
+        "nDOF": m.nDOF, "dt": dt, "J": r["J"],
+        "forward_s": r["t_fwd"], "forward_with_checkpoints_s": r["t_rec"], "adjoint_sweep_s": r["t_adj"],
+        "adjoint_over_forward": r["t_adj"] / r["t_fwd"],
+        "gradient_total_over_forward": (r["t_rec"] + r["t_adj"]) / r["t_fwd"],
         "adjoint_kernels_ms_avg": {k: (v_[0] / v_[1] if v_[1] else 0.0) for k, v_ in ker.items()},
         "adjoint_kernels_ms_total": {k: v_[0] for k, v_ in ker.items()},
-        "device_memory_GB": mem / 1e9,
+        "device_memory_GB": r["mem"] / 1e9,
         "fd_check": {"direction": "seeded N(0,1) x 0.1 mean|U_v| per variable", "dJ_dot_v": exact,
                      "central_fd_rel_err_by_eps": fd, "best": min(fd.values())},
     }
+    if mixed_leg:
+        op, rm = _adjoint_leg(m, U0, W, dt, steps, every, "mixed")
+        op.close()
+        gm = rm["g"]
+        kerm = rm["ker"]
+        out["mixed_forward"] = {
+            "what": "forward and segment recompute in the mixed form (FP32 fluxes), adjoint accumulated in FP64 "
+                    "along that trajectory",
+            "forward_s": rm["t_fwd"], "forward_with_checkpoints_s": rm["t_rec"], "adjoint_sweep_s": rm["t_adj"],
+            "adjoint_over_forward": rm["t_adj"] / rm["t_fwd"],
+            "adjoint_over_fp64_forward": rm["t_adj"] / r["t_fwd"],
+            "gradient_total_over_forward": (rm["t_rec"] + rm["t_adj"]) / rm["t_fwd"],
+            "J_rel_diff_vs_fp64": abs(rm["J"] - r["J"]) / abs(r["J"]),
+            "gradient_rel_l2_vs_fp64_adjoint": float(np.linalg.norm((gm - g0).ravel()) / np.linalg.norm(g0.ravel())),
+            "dJ_dot_v_rel_err_vs_fp64_fd": abs(float(np.sum(gm * v)) - fd_best) / abs(fd_best),
+            "adjoint_kernels_ms_avg": {k: (v_[0] / v_[1] if v_[1] else 0.0) for k, v_ in kerm.items()},
+            "device_memory_GB": rm["mem"] / 1e9,
+        }
+    return out
 
 
-def cpu_baseline(N=7, ne=16):
-    """The FP64 oracle (C/OpenMP restatement of the reference operator) on all host cores, one LSERK step of
-    the same TGV configuration on a 16^3 mesh (same per-DOF work as the headline mesh)."""
+def cpu_baseline(N=7, ne=16, min_wall=1.0, max_steps=20):
+    """The FP64 oracle (C/OpenMP restatement of the reference operator) on the host cores (at most 16, the CPU
+    share of one GPU), LSERK steps of the same TGV configuration on a 16^3 mesh (same per-DOF work as the
+    headline mesh) until at least min_wall seconds have passed (about 16 CPU-seconds on 16 threads)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     m, U = tgv_case(ne, N)
-    cores = os.cpu_count() or 1
+    cores = max(1, min(os.cpu_count() or 1, 16))
     O.set_threads(cores)
     op = O.Operator(m, split=True, riemann=O.RIEMANN_LF, kappa=1.4, R=71.42857, Pr=0.72, mu0=1.0 / 1600.0)
     dt, _e, _c, _v = op.calc_dt(U, TGV_GAS["CFLScale"], TGV_GAS["DFLScale"])
     K = np.zeros_like(U, order="F")
     Uc = np.asfortranarray(U.copy())
     op.rhs(Uc)  # first touch
+    steps = 0
     t0 = time.perf_counter()
-    op.lserk_step(Uc, K, 0.0, dt)
-    t1 = time.perf_counter()
-    return dict(value=m.nDOF * NSTAGES / (t1 - t0), unit="DOF-updates/s", cores=cores, kind="port",
-                sample=f"oracle FP64, one LSERK step (5 stages) of TGV N={N} {ne}^3 ({m.nDOF} DOF), "
-                       f"{t1 - t0:.2f} s wall on {cores} threads",
+    while steps < max_steps and (steps == 0 or time.perf_counter() - t0 < min_wall):
+        op.lserk_step(Uc, K, steps * dt, dt)
+        steps += 1
+    wall = time.perf_counter() - t0
+    return dict(value=m.nDOF * NSTAGES * steps / wall, unit="DOF-updates/s", cores=cores, kind="port",
+                sample=f"oracle FP64, {steps} LSERK steps (5 stages each) of TGV N={N} {ne}^3 ({m.nDOF} DOF), "
+                       f"{wall:.2f} s wall on {cores} threads",
                 cpu=platform.processor() or platform.machine())
 
 

@@ -189,8 +189,8 @@ struct KernelSet {
     int block = 0;
     void (*trace)(hipStream_t, int, const double*, Mesh, void*, GasD) = nullptr;
     void (*lift)(hipStream_t, int, const double*, Mesh, const void*, void*, const void*, GasD, double, double) = nullptr;
-    void (*rhs)(hipStream_t, int, double*, Mesh, const void*, const void*, void*, double*, double*, const void*, GasD,
-                int, double, double, Stage) = nullptr;
+    void (*rhs)(hipStream_t, int, const double*, double*, Mesh, const void*, const void*, void*, double*, double*,
+                const void*, GasD, int, double, double, Stage) = nullptr;
     void (*dt)(hipStream_t, int, const double*, const double*, double, double, double, double, double, double*,
                double*, int*) = nullptr;
     void (*grad)(hipStream_t, int, const double*, Mesh, const void*, const void*, GasD, double, double,
@@ -211,21 +211,23 @@ struct Launch {
                            (const T*)mats, g, T(LHm), T(LHp));
     }
     template <int RIE>
-    static void rhs_r(hipStream_t s, int nE, double* U, Mesh m, const void* utr, const void* vtr, void* utr_next,
-                      double* Ut, double* K, const void* mats, GasD g, double LHm, double LHp, Stage st)
+    static void rhs_r(hipStream_t s, int nE, const double* U, double* Uout, Mesh m, const void* utr, const void* vtr,
+                      void* utr_next, double* Ut, double* K, const void* mats, GasD g, double LHm, double LHp,
+                      Stage st)
     {
-        hipLaunchKernelGGL((k_rhs<N, T, T, SPLIT, RIE>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr,
+        hipLaunchKernelGGL((k_rhs<N, T, T, SPLIT, RIE>), dim3(nE), dim3(D::block), 0, s, U, Uout, m, (const T*)utr,
                            (const T*)vtr, (T*)utr_next, Ut, K, (const T*)mats, g, T(LHm), T(LHp), st);
     }
-    static void rhs(hipStream_t s, int nE, double* U, Mesh m, const void* utr, const void* vtr, void* utr_next,
-                    double* Ut, double* K, const void* mats, GasD g, int riemann, double LHm, double LHp, Stage st)
+    static void rhs(hipStream_t s, int nE, const double* U, double* Uout, Mesh m, const void* utr, const void* vtr,
+                    void* utr_next, double* Ut, double* K, const void* mats, GasD g, int riemann, double LHm,
+                    double LHp, Stage st)
     {
         if (riemann == GALX_RIEMANN_LF)
-            rhs_r<RIEMANN_LF>(s, nE, U, m, utr, vtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
+            rhs_r<RIEMANN_LF>(s, nE, U, Uout, m, utr, vtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
         else if (riemann == GALX_RIEMANN_ROE)
-            rhs_r<RIEMANN_ROE>(s, nE, U, m, utr, vtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
+            rhs_r<RIEMANN_ROE>(s, nE, U, Uout, m, utr, vtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
         else
-            rhs_r<RIEMANN_ROEEF>(s, nE, U, m, utr, vtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
+            rhs_r<RIEMANN_ROEEF>(s, nE, U, Uout, m, utr, vtr, utr_next, Ut, K, mats, g, LHm, LHp, st);
     }
     static void dt(hipStream_t s, int nE, const double* U, const double* geo, double kappa, double mu0, double KPmax,
                    double CFL, double DFL, double* dtc, double* dtv, int* err)
@@ -256,19 +258,22 @@ struct Launch {
 // discrete adjoint (split PI + LF, FP64)
 struct AdjSet {
     bool ok = false;
-    void (*pre)(hipStream_t, int, double*, const double*, double, Mesh, double, double, double*) = nullptr;
+    void (*pre)(hipStream_t, int, double*, double*, double, Mesh, double, double, double*, const double*,
+                const double*, GasD) = nullptr;
     void (*main)(hipStream_t, int, const double*, Mesh, const double*, double*, const double*, const double*, GasD,
                  double, double, double*, int, int, double, double*) = nullptr;
     void (*face)(hipStream_t, int, const double*, Mesh, const double*, GasD, double*) = nullptr;
+    void (*trace64)(hipStream_t, int, const double*, Mesh, double*, GasD) = nullptr;  // FP64 U trace records
 };
 
 template <int N>
 struct AdjLaunch {
     using D = Dm<N>;
-    static void pre(hipStream_t s, int nE, double* kbar, const double* Ubar, double bdt, Mesh m, double LHm,
-                    double LHp, double* mutr)
+    static void pre(hipStream_t s, int nE, double* kbar, double* Ubar, double bdt, Mesh m, double LHm, double LHp,
+                    double* mutr, const double* Uprev, const double* nghtr, GasD g)
     {
-        hipLaunchKernelGGL((k_adj_pre<N>), dim3(nE), dim3(D::block), 0, s, kbar, Ubar, bdt, m, LHm, LHp, mutr);
+        hipLaunchKernelGGL((k_adj_pre<N>), dim3(nE), dim3(D::block), 0, s, kbar, Ubar, bdt, m, LHm, LHp, mutr, Uprev,
+                           nghtr, g);
     }
     static void main(hipStream_t s, int nE, const double* U, Mesh m, const double* utr, double* lam,
                      const double* mutr, const double* mats, GasD g, double LHm, double LHp, double* Ubar, int acc,
@@ -281,6 +286,10 @@ struct AdjLaunch {
     {
         hipLaunchKernelGGL((k_adj_face<N>), dim3(nE), dim3(D::block), 0, s, U, m, nghtr, g, Ubar);
     }
+    static void trace64(hipStream_t s, int nE, const double* U, Mesh m, double* utr, GasD g)
+    {
+        hipLaunchKernelGGL((k_trace<N, double>), dim3(nE), dim3(D::block), 0, s, U, m, utr, g);
+    }
     static AdjSet make()
     {
         AdjSet a;
@@ -288,6 +297,7 @@ struct AdjLaunch {
         a.pre = &pre;
         a.main = &main;
         a.face = &face;
+        a.trace64 = &trace64;
         return a;
     }
 };
@@ -358,7 +368,11 @@ struct galx_ctx {
     double *dKbar = nullptr, *dUbar = nullptr, *dMutr = nullptr, *dNghtr = nullptr;
     std::vector<double*> ckpt;   // U at steps 0, C, 2C, ... of the recorded trajectory
     std::vector<double*> stages; // stage inputs of one segment of C steps (5 per step)
+    std::vector<double*> stageTr; // their FP64 trace records, kept from the recompute (FP64 form)
     double *dUfinal = nullptr;
+    // mixed form: the adjoint kernels run in FP64 on FP64 trace records recomputed from each stage input,
+    // with the FP64 matrices
+    double *dMats64 = nullptr, *dTr64 = nullptr;
     double trajT = 0.0, trajDt = 0.0;
     int trajSteps = 0, trajEvery = 0;
     bool trajValid = false;
@@ -423,6 +437,8 @@ static void flush_timers(galx_ctx* c)
 
 static Mesh mesh_of(galx_ctx* c) { return Mesh{c->dGeo, c->dFgeo, c->dConn, c->dInv}; }
 static GasD gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, c->pref}; }
+// the adjoint kernels and their FP64 trace records use the pressure itself (no offset)
+static GasD adj_gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, 0.0}; }
 
 extern "C" {
 
@@ -469,8 +485,7 @@ int galx_create(const galx_config* cfg, galx_ctx** out)
     c->n2 = c->n * c->n;
     c->n3 = c->n2 * c->n;
     c->ks = ks;
-    if (cfg->precision == GALX_PREC_FP64 && cfg->form == GALX_FORM_SPLIT_PI && cfg->riemann == GALX_RIEMANN_LF)
-        c->as = select_adjoint(cfg->N);
+    if (cfg->form == GALX_FORM_SPLIT_PI && cfg->riemann == GALX_RIEMANN_LF) c->as = select_adjoint(cfg->N);
     c->tsz = cfg->precision == GALX_PREC_FP64 ? 8 : 4;
     c->basis = make_basis(cfg->N);
     // THERMAL_CONDUCTIVITY_EOS (src/equations/navierstokes/idealgas/eos.h:110)
@@ -497,6 +512,10 @@ int galx_create(const galx_config* cfg, galx_ctx** out)
         if ((rc = dalloc(c, (float**)&c->dMats, 3 * nn * sizeof(float)))) { galx_destroy(c); return rc; }
         e = hipMemcpy(c->dMats, mf.data(), 3 * nn * sizeof(float), hipMemcpyHostToDevice);
     }
+    if (e == hipSuccess && c->as.ok) {
+        if ((rc = dalloc(c, &c->dMats64, 3 * nn * sizeof(double)))) { galx_destroy(c); return rc; }
+        e = hipMemcpy(c->dMats64, md.data(), 3 * nn * sizeof(double), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         galx_destroy(c);
         return fail(GALX_E_HIP, "hipMemcpy: %s", hipGetErrorString(e));
@@ -514,10 +533,11 @@ int galx_destroy(galx_ctx* c)
     void* ps[] = {c->dU, c->dK, c->dUt, c->dScratch, c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dUtr[0], c->dUtr[1],
                   c->dVtr, c->dMats, c->dDtc, c->dDtv, c->dDtOut, c->dErr};
     for (void* p : ps) dfree(p);
-    void* pa[] = {c->dKbar, c->dUbar, c->dMutr, c->dNghtr, c->dUfinal, c->dGrad, c->dGradRef};
+    void* pa[] = {c->dKbar, c->dUbar, c->dMutr, c->dNghtr, c->dUfinal, c->dGrad, c->dGradRef, c->dMats64, c->dTr64};
     for (void* p : pa) dfree(p);
     for (double* p : c->ckpt) dfree(p);
     for (double* p : c->stages) dfree(p);
+    for (double* p : c->stageTr) dfree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GALX_OK;
@@ -711,20 +731,25 @@ int galx_get_state(galx_ctx* c, double* U)
     return GALX_OK;
 }
 
-static void launch_lift(galx_ctx* c)
+// lifting of the state U (default: the context state) with its U traces utr (default: the current buffer)
+static void launch_lift(galx_ctx* c, const double* U = nullptr, const void* utr = nullptr)
 {
     TimerScope ts(c, "lift");
     const Basis& B = c->basis;
-    c->ks.lift(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[c->cur], c->dVtr, c->dMats, gas_of(c), 1.0 / B.w[0],
+    c->ks.lift(c->stream, c->nElems, U ? U : c->dU, mesh_of(c), utr ? utr : c->dUtr[c->cur], c->dVtr, c->dMats, gas_of(c), 1.0 / B.w[0],
                1.0 / B.w[c->N]);
 }
 
-static void launch_rhs(galx_ctx* c, Stage st)
+// one RHS pass on the state U (default: the context state) with its traces utr, the updated state to Uout
+// (default: in place) and its traces to utr_next (defaults: the double-buffered context traces)
+static void launch_rhs(galx_ctx* c, Stage st, const double* U = nullptr, double* Uout = nullptr,
+                       const void* utr = nullptr, void* utr_next = nullptr)
 {
     TimerScope ts(c, "rhs");
     const Basis& B = c->basis;
-    c->ks.rhs(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[c->cur], c->dVtr, c->dUtr[c->cur ^ 1], c->dUt, c->dK,
-              c->dMats, gas_of(c), c->cfg.riemann, 1.0 / B.w[0], 1.0 / B.w[c->N], st);
+    c->ks.rhs(c->stream, c->nElems, U ? U : c->dU, Uout ? Uout : c->dU, mesh_of(c), utr ? utr : c->dUtr[c->cur],
+              c->dVtr, utr_next ? utr_next : c->dUtr[c->cur ^ 1], c->dUt, c->dK, c->dMats, gas_of(c),
+              c->cfg.riemann, 1.0 / B.w[0], 1.0 / B.w[c->N], st);
 }
 
 int galx_rhs(galx_ctx* c, double t)
@@ -937,8 +962,7 @@ int galx_memory_bytes(galx_ctx* c, long long* bytes)
 static int adj_check(galx_ctx* c)
 {
     if (!c) return fail(GALX_E_ARG, "null context");
-    if (!c->as.ok)
-        return fail(GALX_E_ARG, "the discrete adjoint is implemented for form SPLIT_PI, riemann LF, precision FP64");
+    if (!c->as.ok) return fail(GALX_E_ARG, "the discrete adjoint is implemented for form SPLIT_PI, riemann LF");
     if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
     HIPCHK(hipSetDevice(c->cfg.device));
     long nDOF = (long)c->nElems * c->n3, nfn = (long)c->nSides * c->n2;
@@ -947,22 +971,36 @@ static int adj_check(galx_ctx* c)
     if ((rc = dalloc(c, &c->dUbar, 5 * nDOF * sizeof(double)))) return rc;
     if ((rc = dalloc(c, &c->dMutr, 2 * 5 * nfn * sizeof(double)))) return rc;
     if ((rc = dalloc(c, &c->dNghtr, 2 * 4 * nfn * sizeof(double)))) return rc;
+    if (c->tsz != 8 && (rc = dalloc(c, &c->dTr64, 2 * 5 * nfn * sizeof(double)))) return rc;
     return GALX_OK;
 }
 
+// FP64 trace records of U for the adjoint kernels: the context's own in the FP64 form (valid for dU), recomputed
+// into dTr64 in the mixed form
+static const double* adj_traces(galx_ctx* c, const double* U, const double* fp64_traces)
+{
+    if (c->tsz == 8) return fp64_traces;
+    c->as.trace64(c->stream, c->nElems, U, mesh_of(c), c->dTr64, adj_gas_of(c));
+    return c->dTr64;
+}
+
 // Ubar (+)= (dR/dU(U))^T lam for U in element planes with its FP64 trace records utr; lam is consumed:
-// lam <- lam + bdt*Ubar first when Ubar accumulates (LSERK stage), and lam <- a*lam at the end when kscale
+// lam <- lam + bdt*Ubar first when Ubar accumulates (LSERK stage), and lam <- a*lam at the end when kscale.
+// Uprev: state of the previous VJP whose neighbour lifting term is still pending (completed first, fused);
+// finish: apply this VJP's own neighbour lifting term now (else it stays pending for the next call).
 static void adj_apply(galx_ctx* c, const double* U, const double* utr, double* lam, double* Ubar, int accumulate,
-                      double bdt, int kscale, double a)
+                      double bdt, int kscale, double a, const double* Uprev, bool finish)
 {
     TimerScope ts(c, "adjoint");
     const Basis& B = c->basis;
     double LHm = 1.0 / B.w[0], LHp = 1.0 / B.w[c->N];
     Mesh m = mesh_of(c);
-    c->as.pre(c->stream, c->nElems, lam, accumulate ? Ubar : nullptr, bdt, m, LHm, LHp, c->dMutr);
-    c->as.main(c->stream, c->nElems, U, m, utr, lam, c->dMutr, (const double*)c->dMats, gas_of(c), LHm, LHp, Ubar,
-               accumulate, kscale, a, c->dNghtr);
-    c->as.face(c->stream, c->nElems, U, m, c->dNghtr, gas_of(c), Ubar);
+    const GasD g = adj_gas_of(c);
+    c->as.pre(c->stream, c->nElems, lam, accumulate ? Ubar : nullptr, bdt, m, LHm, LHp, c->dMutr, Uprev, c->dNghtr,
+              g);
+    c->as.main(c->stream, c->nElems, U, m, utr, lam, c->dMutr, c->dMats64, g, LHm, LHp, Ubar, accumulate, kscale, a,
+               c->dNghtr);
+    if (finish) c->as.face(c->stream, c->nElems, U, m, c->dNghtr, g, Ubar);
 }
 
 int galx_rhs_vjp_device(galx_ctx* c, const double* dLam, double* dUbarOut)
@@ -974,7 +1012,8 @@ int galx_rhs_vjp_device(galx_ctx* c, const double* dLam, double* dUbarOut)
     unsigned g = (unsigned)((nDOF + 255) / 256);
     k_ref_to_planes<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, dLam, c->dKbar);
     HIPCHK(hipGetLastError());
-    adj_apply(c, c->dU, (const double*)c->dUtr[c->cur], c->dKbar, c->dUbar, 0, 0.0, 0, 0.0);
+    adj_apply(c, c->dU, adj_traces(c, c->dU, (const double*)c->dUtr[c->cur]), c->dKbar, c->dUbar, 0, 0.0, 0, 0.0,
+              nullptr, true);
     HIPCHK(hipGetLastError());
     k_planes_to_ref<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUbar, dUbarOut);
     HIPCHK(hipGetLastError());
@@ -993,7 +1032,8 @@ int galx_rhs_vjp(galx_ctx* c, const double* lam, double* UbarOut)
     HIPCHK(hipMemcpyAsync(c->dScratch, lam, 5 * nDOF * sizeof(double), hipMemcpyHostToDevice, c->stream));
     k_ref_to_planes<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dScratch, c->dKbar);
     HIPCHK(hipGetLastError());
-    adj_apply(c, c->dU, (const double*)c->dUtr[c->cur], c->dKbar, c->dUbar, 0, 0.0, 0, 0.0);
+    adj_apply(c, c->dU, adj_traces(c, c->dU, (const double*)c->dUtr[c->cur]), c->dKbar, c->dUbar, 0, 0.0, 0, 0.0,
+              nullptr, true);
     HIPCHK(hipGetLastError());
     k_planes_to_ref<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUbar, c->dScratch);
     HIPCHK(hipGetLastError());
@@ -1032,6 +1072,21 @@ int galx_lserk_forward_ckpt(galx_ctx* c, double t, double dt, int nSteps, int ck
             return fail(rc, "stage storage (%d x %zu bytes; lower ckptEvery): %s", nStage, ub, g_err.c_str());
         c->stages.push_back(p);
     }
+    // FP64 form: the recompute keeps every stage's trace records for the VJPs (the mixed form recomputes FP64
+    // records per VJP instead, its forward traces being FP32)
+    const size_t tb = 2L * 5 * c->nSides * c->n2 * sizeof(double);
+    const int nStageTr = c->tsz == 8 ? nStage : 0;
+    while ((int)c->stageTr.size() > nStageTr) {
+        dfree(c->stageTr.back());
+        c->bytes -= (long long)tb;
+        c->stageTr.pop_back();
+    }
+    while ((int)c->stageTr.size() < nStageTr) {
+        double* p = nullptr;
+        if ((rc = dalloc(c, &p, tb)))
+            return fail(rc, "stage trace storage (%d x %zu bytes; lower ckptEvery): %s", nStage, tb, g_err.c_str());
+        c->stageTr.push_back(p);
+    }
     if ((rc = dalloc(c, &c->dUfinal, ub))) return rc;
     c->trajValid = false;
     for (int seg = 0; seg < nSeg; seg++) {
@@ -1060,32 +1115,44 @@ static int adj_reverse(galx_ctx* c)
     const int nSeg = (nSteps + C - 1) / C;
     Mesh m = mesh_of(c);
     GasD gd = gas_of(c);
-    double* utrA = (double*)c->dUtr[0];
+    const double* pending = nullptr;  // state of the last VJP whose neighbour lifting term is still to be added
     HIPCHK(hipMemsetAsync(c->dKbar, 0, ub, c->stream));
     for (int seg = nSeg - 1; seg >= 0; seg--) {
         const int s0 = seg * C, n = std::min(C, nSteps - s0);
-        // recompute the segment, keeping every stage input
-        HIPCHK(hipMemcpyAsync(c->dU, c->ckpt[seg], ub, hipMemcpyDeviceToDevice, c->stream));
-        c->cur = 0;
-        c->ks.trace(c->stream, c->nElems, c->dU, m, c->dUtr[0], gd);
+        // recompute the segment from its checkpoint; each stage reads its input and writes the next stage's input
+        // (stage 0 of the first step reads the checkpoint itself).  FP64 form: each stage also writes its trace
+        // records into their own buffer, kept for the VJPs; mixed form: the FP32 records alternate between the
+        // context's two trace buffers, and the VJPs recompute FP64 records from the stage inputs.
+        const bool keep = c->tsz == 8;
+        c->ks.trace(c->stream, c->nElems, c->ckpt[seg], m, keep ? (void*)c->stageTr[0] : c->dUtr[0], gd);
         for (int s = 0; s < n; s++)
             for (int stage = 0; stage < 5; stage++) {
-                HIPCHK(hipMemcpyAsync(c->stages[5 * s + stage], c->dU, ub, hipMemcpyDeviceToDevice, c->stream));
-                launch_lift(c);
+                const int idx = 5 * s + stage;
+                const bool last = idx + 1 == 5 * n;
+                const double* Ui = idx == 0 ? c->ckpt[seg] : c->stages[idx];
+                double* Uo = last ? c->dU : c->stages[idx + 1];
+                void* tri = keep ? (void*)c->stageTr[idx] : c->dUtr[idx & 1];
+                void* tro = keep ? (last ? c->dUtr[0] : (void*)c->stageTr[idx + 1]) : c->dUtr[(idx + 1) & 1];
+                launch_lift(c, Ui, tri);
                 double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
-                launch_rhs(c, Stage{mRKA, RKB[stage] * c->trajDt, 1});
-                c->cur ^= 1;
+                launch_rhs(c, Stage{mRKA, RKB[stage] * c->trajDt, 1}, Ui, Uo, tri, tro);
             }
         HIPCHK(hipGetLastError());
         // reverse
         for (int s = n - 1; s >= 0; s--)
             for (int stage = 4; stage >= 0; stage--) {
-                const double* Us = c->stages[5 * s + stage];
-                c->ks.trace(c->stream, c->nElems, Us, m, utrA, gd);
+                const int idx = 5 * s + stage;
+                const double* Us = idx == 0 ? c->ckpt[seg] : c->stages[idx];
                 double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
-                adj_apply(c, Us, utrA, c->dKbar, c->dUbar, 1, RKB[stage] * c->trajDt, 1, mRKA);
+                const double* tr = adj_traces(c, Us, keep ? c->stageTr[idx] : nullptr);
+                adj_apply(c, Us, tr, c->dKbar, c->dUbar, 1, RKB[stage] * c->trajDt, 1, mRKA, pending, false);
+                pending = Us;  // a checkpoint or a stage buffer of this segment, read before it is overwritten
             }
         HIPCHK(hipGetLastError());
+    }
+    if (pending) {
+        TimerScope ts(c, "adjoint");
+        c->as.face(c->stream, c->nElems, pending, m, c->dNghtr, adj_gas_of(c), c->dUbar);
     }
     // back to the final state of the trajectory
     HIPCHK(hipMemcpyAsync(c->dU, c->dUfinal, ub, hipMemcpyDeviceToDevice, c->stream));

@@ -25,6 +25,9 @@
 #ifndef GALX_UNROLL_SPLIT
 #define GALX_UNROLL_SPLIT 1
 #endif
+#ifndef GALX_MIXED_WAVES
+#define GALX_MIXED_WAVES 6
+#endif
 
 namespace galx {
 
@@ -115,16 +118,17 @@ __device__ __forceinline__ void load_mats(Mats<N, T>& M, const T* __restrict__ g
 // nVarLifting=5 never enters the fluxes and is not formed.
 //   s_lift  : T[4][n3]   own (u,v,w,T), filled by the caller before the call (synchronised)
 //   s_lflux : T[6][7][n2] scratch
+//   s_nb    : T[6][5][n2] (optional) receives the neighbour trace records of the six faces, for later face tasks
 // returns g[12] = grad(u,v,w,T) in x, then y, then z for node tid (valid if tid < n3).
 // ---------------------------------------------------------------------------------------------------------
+// BR1 lifting flux 0.5*SurfElem*(-L_master + L_slave) and the normal of every face node of element e
+// (the face-task half of lift_gradient); s_lflux [6][7][n2], s_nb as in lift_gradient
 template <int N, typename T, typename TT>
-__device__ __forceinline__ void lift_gradient(int e, int tid, const Mesh& mesh, const TT* __restrict__ utr,
-                                              const Mats<N, T>& M, const T* s_lift, T* s_lflux, const Gas<T>& gas,
-                                              T LHm, T LHp, T* g)
+__device__ __forceinline__ void lift_face_fluxes(int e, int tid, const Mesh& mesh, const TT* __restrict__ utr,
+                                                 const T* s_lift, T* s_lflux, const Gas<T>& gas, T* s_nb)
 {
     using D = Dm<N>;
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
-    // face tasks: BR1 lifting flux 0.5*SurfElem*(-L_master + L_slave) and the normal
     for (int t = tid; t < D::nf; t += D::block) {
         int ls = t / n2 + 1, ab = t - (ls - 1) * n2, a = ab % n, b = ab / n;
         int node = face_vol_node<N>(ls, a, b);
@@ -136,6 +140,10 @@ __device__ __forceinline__ void lift_gradient(int e, int tid, const Mesh& mesh, 
         T tr[5], uc[5], pr[6];
 #pragma unroll
         for (int v = 0; v < 5; v++) tr[v] = T(nb[v * n2]);
+        if (s_nb) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) s_nb[(ls - 1) * 5 * n2 + v * n2 + ab] = tr[v];
+        }
         trace_state(tr, gas, uc, pr);
         T Lnb[4] = {pr[1], pr[2], pr[3], pr[5]};
         const double* fg = mesh.fgeo + side * 10 * n2 + pq;
@@ -151,53 +159,69 @@ __device__ __forceinline__ void lift_gradient(int e, int tid, const Mesh& mesh, 
         o[5 * n2] = T(fg[n2]);
         o[6 * n2] = T(fg[2 * n2]);
     }
-    __syncthreads();
-    if (tid < n3) {
-        int i = tid % n, j = (tid / n) % n, k = tid / n2;
-        T gxi[4] = {0, 0, 0, 0}, get[4] = {0, 0, 0, 0}, gze[4] = {0, 0, 0, 0};
+}
+
+// BR1 gradient g[12] = grad(u,v,w,T) (x, then y, then z) at volume node `node` of element e, from s_lift and the
+// lifting fluxes of lift_face_fluxes (the node half of lift_gradient; any thread may evaluate any node)
+template <int N, typename T>
+__device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh, const Mats<N, T>& M,
+                                              const T* s_lift, const T* s_lflux, T LHm, T LHp, T* g)
+{
+    constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+    const int i = node % n, j = (node / n) % n, k = node / n2;
+    T gxi[4] = {0, 0, 0, 0}, get[4] = {0, 0, 0, 0}, gze[4] = {0, 0, 0, 0};
 #pragma unroll GALX_UNROLL_L
-        for (int l = 0; l <= N; l++) {
-            T di = M.DT[l + n * i], dj = M.DT[l + n * j], dk = M.DT[l + n * k];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                gxi[q] = gxi[q] + di * s_lift[q * n3 + l + n * (j + n * k)];
-                get[q] = get[q] + dj * s_lift[q * n3 + i + n * (l + n * k)];
-                gze[q] = gze[q] + dk * s_lift[q * n3 + i + n * (j + n * l)];
-            }
-        }
-        const double* ge = mesh.geo + (long)e * 10 * n3 + tid;
-        T mf[3] = {T(ge[0]), T(ge[n3]), T(ge[2 * n3])};
-        T mg[3] = {T(ge[3 * n3]), T(ge[4 * n3]), T(ge[5 * n3])};
-        T mh[3] = {T(ge[6 * n3]), T(ge[7 * n3]), T(ge[8 * n3])};
-        T sJ = T(ge[9 * n3]);
-        T loc[12];
-#pragma unroll
-        for (int q = 0; q < 12; q++) loc[q] = T(0);
-#pragma unroll
-        for (int ls = 1; ls <= 6; ls++) {
-            int ab;
-            if (!on_face<N>(ls, i, j, k, ab)) continue;
-            const T* o = s_lflux + (ls - 1) * 7 * n2 + ab;
-            T LH = (ls == 1 || ls == 2 || ls == 5) ? LHm : LHp;
-            T n0 = o[4 * n2], n1 = o[5 * n2], n2v = o[6 * n2];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                T lf = o[q * n2];
-                loc[q] = loc[q] + T(1) * (lf * n0) * LH;
-                loc[4 + q] = loc[4 + q] + T(1) * (lf * n1) * LH;
-                loc[8 + q] = loc[8 + q] + T(1) * (lf * n2v) * LH;
-            }
-        }
+    for (int l = 0; l <= N; l++) {
+        T di = M.DT[l + n * i], dj = M.DT[l + n * j], dk = M.DT[l + n * k];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            T vx = mf[0] * gxi[q] + mg[0] * get[q] + mh[0] * gze[q];
-            T vy = mf[1] * gxi[q] + mg[1] * get[q] + mh[1] * gze[q];
-            T vz = mf[2] * gxi[q] + mg[2] * get[q] + mh[2] * gze[q];
-            g[q] = sJ * (vx + loc[q]);
-            g[4 + q] = sJ * (vy + loc[4 + q]);
-            g[8 + q] = sJ * (vz + loc[8 + q]);
+            gxi[q] = gxi[q] + di * s_lift[q * n3 + l + n * (j + n * k)];
+            get[q] = get[q] + dj * s_lift[q * n3 + i + n * (l + n * k)];
+            gze[q] = gze[q] + dk * s_lift[q * n3 + i + n * (j + n * l)];
         }
     }
+    const double* ge = mesh.geo + (long)e * 10 * n3 + node;
+    T mf[3] = {T(ge[0]), T(ge[n3]), T(ge[2 * n3])};
+    T mg[3] = {T(ge[3 * n3]), T(ge[4 * n3]), T(ge[5 * n3])};
+    T mh[3] = {T(ge[6 * n3]), T(ge[7 * n3]), T(ge[8 * n3])};
+    T sJ = T(ge[9 * n3]);
+    T loc[12];
+#pragma unroll
+    for (int q = 0; q < 12; q++) loc[q] = T(0);
+#pragma unroll
+    for (int ls = 1; ls <= 6; ls++) {
+        int ab;
+        if (!on_face<N>(ls, i, j, k, ab)) continue;
+        const T* o = s_lflux + (ls - 1) * 7 * n2 + ab;
+        T LH = (ls == 1 || ls == 2 || ls == 5) ? LHm : LHp;
+        T n0 = o[4 * n2], n1 = o[5 * n2], n2v = o[6 * n2];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            T lf = o[q * n2];
+            loc[q] = loc[q] + T(1) * (lf * n0) * LH;
+            loc[4 + q] = loc[4 + q] + T(1) * (lf * n1) * LH;
+            loc[8 + q] = loc[8 + q] + T(1) * (lf * n2v) * LH;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        T vx = mf[0] * gxi[q] + mg[0] * get[q] + mh[0] * gze[q];
+        T vy = mf[1] * gxi[q] + mg[1] * get[q] + mh[1] * gze[q];
+        T vz = mf[2] * gxi[q] + mg[2] * get[q] + mh[2] * gze[q];
+        g[q] = sJ * (vx + loc[q]);
+        g[4 + q] = sJ * (vy + loc[4 + q]);
+        g[8 + q] = sJ * (vz + loc[8 + q]);
+    }
+}
+
+template <int N, typename T, typename TT>
+__device__ __forceinline__ void lift_gradient(int e, int tid, const Mesh& mesh, const TT* __restrict__ utr,
+                                              const Mats<N, T>& M, const T* s_lift, T* s_lflux, const Gas<T>& gas,
+                                              T LHm, T LHp, T* g, T* s_nb = nullptr)
+{
+    lift_face_fluxes<N, T, TT>(e, tid, mesh, utr, s_lift, s_lflux, gas, s_nb);
+    __syncthreads();
+    if (tid < Dm<N>::n3) node_gradient<N, T>(e, tid, mesh, M, s_lift, s_lflux, LHm, LHp, g);
 }
 
 // volume state of node tid: ConsToPrim (eos.f90:212-239) in FP64, then shifted by the pressure offset and
@@ -237,31 +261,6 @@ __device__ __forceinline__ void store_trace(TT* o, int n2, const double* c, cons
     for (int v = 0; v < 5; v++) o[v * n2] = TT(tr[v]);
 }
 
-// normal viscous flux F_v(P, g).n_master of node (i,j,k) on each of its faces -> s_fv[6][4][n2] (components
-// rho u, rho v, rho w, E; EvalNormalDiffFlux3D flux.f90:779-847)
-template <int N, typename T>
-__device__ __forceinline__ void store_face_viscous(int e, int i, int j, int k, const Mesh& mesh, const T* pr,
-                                                   const T* g, const Gas<T>& gas, T* s_fv)
-{
-    constexpr int n2 = (N + 1) * (N + 1);
-#pragma unroll
-    for (int ls = 1; ls <= 6; ls++) {
-        int ab;
-        if (!on_face<N>(ls, i, j, k, ab)) continue;
-        int c = mesh.conn[e * 6 + ls - 1];
-        long side = c >> 3;
-        int flip = c & 7;
-        int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
-        const double* fg = mesh.fgeo + side * 10 * n2 + pq;
-        T nv[3] = {T(fg[0]), T(fg[n2]), T(fg[2 * n2])};
-        T f[5];
-        normal_diff_flux(pr, g, nv, gas, f);
-        T* o = s_fv + (ls - 1) * 4 * n2 + ab;
-#pragma unroll
-        for (int q = 0; q < 4; q++) o[q * n2] = f[1 + q];
-    }
-}
-
 // ---------------------------------------------------------------------------------------------------------
 // k_trace: U traces of all faces (ProlongToFaceCons, EvalElemFaceGL_Device prolongtoface.t90:268-344, in the
 // element-wise form: each element writes its own slot of each of its 6 sides).
@@ -292,7 +291,9 @@ __global__ void __launch_bounds__(Dm<N>::block)
 // ---------------------------------------------------------------------------------------------------------
 // k_lift: BR1 gradients and, on the element's own face nodes, the normal viscous flux
 // F_v(U, grad U).n_master (EvalNormalDiffFlux3D flux.f90:779-847 as ViscousFlux riemann.f90:638-700 applies it)
-// written to the element's slot of the viscous-flux traces.
+// written to the element's slot of the viscous-flux traces; one face task per face node forms it.  Every node
+// thread forms its gradient (eight waves in parallel), and the face-node gradients reach the face tasks through
+// LDS: measured faster than face tasks that each evaluate their node's gradient (fewer, longer waves).
 // ---------------------------------------------------------------------------------------------------------
 template <int N, typename T, typename TT>
 __global__ void __launch_bounds__(Dm<N>::block)
@@ -301,8 +302,7 @@ __global__ void __launch_bounds__(Dm<N>::block)
 {
     using D = Dm<N>;
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
-    constexpr int lds_a = 4 * n3 + 6 * 7 * n2, lds_b = 6 * 4 * n2;
-    __shared__ T s_buf[lds_a > lds_b ? lds_a : lds_b];
+    __shared__ T s_buf[4 * n3 + 6 * 12 * n2];  // own (u,v,w,T) | lifting fluxes, then own face gradients
     __shared__ Mats<N, T> M;
     const int e = blockIdx.x, tid = threadIdx.x;
     T* s_lift = s_buf;
@@ -321,22 +321,38 @@ __global__ void __launch_bounds__(Dm<N>::block)
     T g[12];
     lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_lift, s_lflux, gas, LHm, LHp, g);
     __syncthreads();
-    T* s_out = s_buf;  // [6][4][n2]
+    T* s_g = s_buf + 4 * n3;  // [6][12][n2]
     if (tid < n3) {
         int i = tid % n, j = (tid / n) % n, k = tid / n2;
-        store_face_viscous<N, T>(e, i, j, k, mesh, pr, g, gas, s_out);
+#pragma unroll
+        for (int ls = 1; ls <= 6; ls++) {
+            int ab;
+            if (!on_face<N>(ls, i, j, k, ab)) continue;
+            T* o = s_g + (ls - 1) * 12 * n2 + ab;
+#pragma unroll
+            for (int q = 0; q < 12; q++) o[q * n2] = g[q];
+        }
     }
     __syncthreads();
     for (int t = tid; t < D::nf; t += D::block) {
-        int ls = t / n2 + 1, ab = t - (ls - 1) * n2;
+        int ls = t / n2 + 1, ab = t - (ls - 1) * n2, a = ab % n, b = ab / n;
+        int node = face_vol_node<N>(ls, a, b);
         int c = mesh.conn[e * 6 + ls - 1];
         long side = c >> 3;
         int flip = c & 7;
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
-        TT* o = vtr + ((side * 2 + (flip ? 1 : 0)) * 4) * n2 + pq;
-        const T* s = s_out + (ls - 1) * 4 * n2 + ab;
+        const double* fg = mesh.fgeo + side * 10 * n2 + pq;
+        T nv[3] = {T(fg[0]), T(fg[n2]), T(fg[2 * n2])};
+        T P[4] = {T(0), s_lift[node], s_lift[n3 + node], s_lift[2 * n3 + node]};
+        T G[12];
+        const T* sg = s_g + (ls - 1) * 12 * n2 + ab;
 #pragma unroll
-        for (int q = 0; q < 4; q++) o[q * n2] = TT(s[q * n2]);
+        for (int q = 0; q < 12; q++) G[q] = sg[q * n2];
+        T f[5];
+        normal_diff_flux(P, G, nv, gas, f);
+        TT* o = vtr + ((side * 2 + (flip ? 1 : 0)) * 4) * n2 + pq;
+#pragma unroll
+        for (int q = 0; q < 4; q++) o[q * n2] = TT(f[1 + q]);
     }
 }
 
@@ -381,9 +397,11 @@ __global__ void __launch_bounds__(Dm<N>::block)
 //   else : VolInt_weakForm with EvalTransformedEulerDiffFlux3D (src/dg/volint.f90:129-188)
 //   FillFlux: Riemann + ViscousFlux + SurfElem (src/dg/fillflux.f90:45-172), SurfInt_Device (surfint.t90:352-586)
 //   Ut = -sJ*Ut (src/dg/dg.f90:413,423)
-// LDS: region W (work: lifting, volume fluxes, split-form partner data, face results, new traces) and region V
-// (the own normal viscous fluxes on the six faces, formed right after the lifting and read by the face-flux
-// tasks), so that the gradient registers die before the volume phases.
+// LDS: one work region (lifting, volume fluxes, split-form partner data, face results, new traces).  The face
+// tasks take the normal viscous flux of both sides from the viscous-flux traces (the own slot written by this
+// element's k_lift), so the gradient registers die after the stress tensor is formed.
+// U is read and the new state written to Uout, which may be U itself (in-place LSERK update) or another
+// buffer (the adjoint's recompute keeps every stage input).
 // ---------------------------------------------------------------------------------------------------------
 template <int N, typename T, typename TT, bool SPLIT, int RIEMANN>
 __device__ __forceinline__ void face_flux(const T* tro, const T* fvo, const T* trn, const T* fvn,
@@ -393,10 +411,14 @@ __device__ __forceinline__ void face_flux(const T* tro, const T* fvo, const T* t
     trace_state(tro, gas, Uo, Po);
     trace_state(trn, gas, Un, Pn);
     T nv[3] = {T(fg[0]), T(fg[n2]), T(fg[2 * n2])};
-    T t1[3] = {T(fg[3 * n2]), T(fg[4 * n2]), T(fg[5 * n2])};
-    T t2[3] = {T(fg[6 * n2]), T(fg[7 * n2]), T(fg[8 * n2])};
     const T *UL = flip ? Un : Uo, *UR = flip ? Uo : Un, *PL = flip ? Pn : Po, *PR = flip ? Po : Pn;
-    riemann_point<T, SPLIT>(RIEMANN, UL, UR, PL, PR, nv, t1, t2, gas, F);
+    if (SPLIT && RIEMANN == RIEMANN_LF) {
+        lf_pi_cartesian<T>(UL, UR, PL, PR, nv, gas, F);
+    } else {
+        T t1[3] = {T(fg[3 * n2]), T(fg[4 * n2]), T(fg[5 * n2])};
+        T t2[3] = {T(fg[6 * n2]), T(fg[7 * n2]), T(fg[8 * n2])};
+        riemann_point<T, SPLIT>(RIEMANN, UL, UR, PL, PR, nv, t1, t2, gas, F);
+    }
     // ViscousFlux (riemann.f90:638-700): 0.5*(F_v(master) + F_v(slave)).n
     const T* fvm = flip ? fvn : fvo;
     const T* fvs = flip ? fvo : fvn;
@@ -412,12 +434,12 @@ __device__ __forceinline__ void face_flux(const T* tro, const T* fvo, const T* t
 template <int N, int RIEMANN, typename T>
 struct RhsWaves {
     static constexpr int value =
-        (RIEMANN == RIEMANN_ROEEF || Dm<N>::block < 256) ? 1 : (sizeof(T) == 4 ? 6 : 4);
+        (RIEMANN == RIEMANN_ROEEF || Dm<N>::block < 256) ? 1 : (sizeof(T) == 4 ? GALX_MIXED_WAVES : 4);
 };
 
 template <int N, typename T, typename TT, bool SPLIT, int RIEMANN>
 __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value))
-    k_rhs(double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, const TT* __restrict__ vtr,
+    k_rhs(const double* U, double* Uout, Mesh mesh, const TT* __restrict__ utr, const TT* __restrict__ vtr,
           TT* __restrict__ utr_next, double* __restrict__ Ut, double* __restrict__ Kreg, const T* __restrict__ mats,
           GasD gd, T LHm, T LHp, Stage st)
 {
@@ -429,7 +451,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     constexpr int w_ac = w_a > w_c ? w_a : w_c;
     constexpr int lds_w = w_ac > w_d ? w_ac : w_d;
     __shared__ T s_w[lds_w];
-    __shared__ T s_fv[6 * 4 * n2];
+    __shared__ T s_nb[6 * 5 * n2];  // neighbour trace records, from the lifting to the face tasks
     __shared__ Mats<N, T> M;
     const int e = blockIdx.x, tid = threadIdx.x;
     const bool active = tid < n3;
@@ -448,11 +470,8 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     Stress<T> s;
     {
         T g[12];
-        lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3, gas, LHm, LHp, g);
-        if (active) {
-            stress(pr, g, gas, s);
-            store_face_viscous<N, T>(e, i, j, k, mesh, pr, g, gas, s_fv);
-        }
+        lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3, gas, LHm, LHp, g, s_nb);
+        if (active) stress(pr, g, gas, s);
     }
     __syncthreads();
 
@@ -526,9 +545,11 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll
                     for (int v = 1; v < 5; v++) ut[v] = ut[v] + dh * s_f[(v - 1) * n3 + base + l * stride];
                 }
+                // all l, the diagonal included as in VolInt_splitForm_Kernel (volint.f90:307-317): DVolSurf(i,i)
+                // vanishes for Gauss-Lobatto up to rounding (dg.f90:229-233), and a uniform loop keeps the
+                // wave free of per-lane branches
 #pragma unroll GALX_UNROLL_SPLIT
                 for (int l = 0; l <= N; l++) {
-                    if (l == idx) continue;  // DVolSurf diagonal vanishes for Gauss-Lobatto (src/dg/dg.f90:229-237)
                     int o = base + l * stride;
                     T Mo[3] = {s_m[o], s_m[n3 + o], s_m[2 * n3 + o]};
                     T F[5];
@@ -565,14 +586,14 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll
             for (int v = 0; v < 5; v++) Uo[v] = T(TT(tr[v]));  // exactly the record the neighbour reads
         }
-        const T* so = s_fv + (ls - 1) * 4 * n2 + ab;
+        const TT* vo = vtr + ((side * 2 + (flip ? 1 : 0)) * 4) * n2 + pq;  // written by k_lift of this element
 #pragma unroll
-        for (int q = 0; q < 4; q++) fvo[q] = so[q * n2];
+        for (int q = 0; q < 4; q++) fvo[q] = T(vo[q * n2]);
         const long nslot = side * 2 + (flip ? 0 : 1);
-        const TT* un = utr + nslot * 5 * n2 + pq;
+        const T* un = s_nb + (ls - 1) * 5 * n2 + ab;
         const TT* vn = vtr + nslot * 4 * n2 + pq;
 #pragma unroll
-        for (int v = 0; v < 5; v++) Un[v] = T(un[v * n2]);
+        for (int v = 0; v < 5; v++) Un[v] = un[v * n2];
 #pragma unroll
         for (int q = 0; q < 4; q++) fvn[q] = T(vn[q * n2]);
         T F[5];
@@ -595,7 +616,8 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         }
         double sJd = ge[9 * n3];
         double* ut_out = (st.mode == 0 ? Ut : Kreg) + (long)e * 5 * n3 + tid;
-        double* uo = U + (long)e * 5 * n3 + tid;
+        const double* uo = U + (long)e * 5 * n3 + tid;
+        double* un = Uout + (long)e * 5 * n3 + tid;
 #pragma unroll
         for (int v = 0; v < 5; v++) {
             T tot = ut[v] + loc[v];
@@ -606,7 +628,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                 double kk = ut_out[v * n3] * st.mRKA + utd;
                 ut_out[v * n3] = kk;
                 unew[v] = uo[v * n3] + kk * st.bdt;
-                uo[v * n3] = unew[v];
+                un[v * n3] = unew[v];
             }
         }
     }

@@ -230,6 +230,30 @@ __device__ __forceinline__ void riemann_point(int solver, const T* UL, const T* 
     Fout[4] = F[4];
 }
 
+// Riemann_LF with the SplitSurfaceFluxPI central part (riemann.f90:707-733, splitflux.f90:735-761) for the
+// Cartesian flux F.n, without the normal-frame rotation of the driver (riemann.f90:216-289): rotating into
+// (n,t1,t2) and back leaves the PI momentum flux 1/8 {rho} {u_n} {u} + 1/2 {p} n with {x} = x_L + x_R and the
+// dissipation -lmax/2 (U_R - U_L) unchanged, so only u.n is needed (algebraically identical to riemann_point).
+template <typename T>
+__device__ __forceinline__ void lf_pi_cartesian(const T* UL, const T* UR, const T* PL, const T* PR, const T* nv,
+                                                const Gas<T>& g, T* F)
+{
+    const T vnL = PL[1] * nv[0] + PL[2] * nv[1] + PL[3] * nv[2];
+    const T vnR = PR[1] * nv[0] + PR[2] * nv[1] + PR[3] * nv[2];
+    const T sRhoL = T(1) / UL[0], sRhoR = T(1) / UR[0];
+    const T HL = (UL[4] + PL[4] + g.hshift) * sRhoL, HR = (UR[4] + PR[4] + g.hshift) * sRhoR;
+    const T cL = gsqrt(g.kappa * (PL[4] + g.pref) * sRhoL), cR = gsqrt(g.kappa * (PR[4] + g.pref) * sRhoR);
+    const T lmax = gmax(gabs(vnR), gabs(vnL)) + gmax(cL, cR);
+    const T sr = UL[0] + UR[0], su = vnL + vnR, q = T(0.125) * sr * su, sp = T(0.5) * (PL[4] + PR[4]);
+    F[0] = T(2) * q;
+#pragma unroll
+    for (int d = 0; d < 3; d++) F[1 + d] = q * (PL[1 + d] + PR[1 + d]) + sp * nv[d];
+    F[4] = q * (HL + HR);
+    const T hl = T(0.5) * lmax;
+#pragma unroll
+    for (int v = 0; v < 5; v++) F[v] = F[v] - hl * (UR[v] - UL[v]);
+}
+
 // gradients of the lifted variables in the order used here: g[4*dir + c], dir in {x,y,z}, c in {u,v,w,T}
 // Shear stress and heat flux terms shared by EvalNormalDiffFlux3D (flux.f90:779-847) and
 // EvalTransformed(Euler)DiffFlux3D (flux.f90:509-700).

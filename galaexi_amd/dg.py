@@ -133,7 +133,7 @@ class DGOperator:
         check(lib().galx_calc_dt(self._h, C.byref(dt), C.byref(err)), "galx_calc_dt")
         return dt.value, err.value
 
-    # discrete adjoint (split PI + LF, FP64)
+    # discrete adjoint (split PI + LF; FP64 VJPs, along a mixed-precision trajectory in a mixed context)
     def rhs_vjp(self, lam):
         """Ubar = (dR/dU)^T lam at the current state (host arrays in the U layout)."""
         lam = _F(lam)

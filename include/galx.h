@@ -121,7 +121,9 @@ int galx_surf_int(int device, int nVar, int N, int nElems, int nSides, const int
                   const double *Flux_master, const double *Flux_slave, double L_HatMinus0, double L_HatPlusN,
                   double *Ut);
 
-/* Discrete adjoint (form SPLIT_PI, riemann LF, precision FP64; GALX_E_ARG otherwise).  The reference has no
+/* Discrete adjoint (form SPLIT_PI, riemann LF; GALX_E_ARG otherwise).  The VJPs are FP64 in both precisions;
+ * a mixed-precision context records and recomputes its trajectory in the mixed form and accumulates the adjoint
+ * in FP64 along it.  The reference has no
  * adjoint; its nearest interface is the matrix-free Jacobian-vector product of the implicit module,
  * MatrixVector (src/implicit/implicit.f90:480-528), which forms (R(U+eps v) - R(U))/eps by finite differences.
  * These entry points give the exact transpose of that Jacobian: Ubar = (dR/dU)^T lam, to rounding.

@@ -1,4 +1,5 @@
-"""GPU tests of the discrete adjoint (split PI + LF, BR1, FP64) through the C ABI.
+"""GPU tests of the discrete adjoint (split PI + LF, BR1, FP64; also along a mixed-precision forward) through the C
+ABI.
 
 Bar (stated per test): the vector-Jacobian product agrees with central finite differences of the FP64 oracle's
 operator to 1e-6 relative (best over a step-size sweep), with the reference's matrix-free Jacobian-vector
@@ -326,7 +327,7 @@ def test_adjoint_api_errors():
 
     m, U = noisy_case((2, 2, 2), 3, seed=0, curved=False, rot=False)
     for kw in (dict(form="weak", riemann="lf", precision="fp64"), dict(form="split", riemann="roe", precision="fp64"),
-               dict(form="split", riemann="lf", precision="mixed")):
+               dict(form="split", riemann="roeentropyfix", precision="mixed")):
         op = dg.DGOperator(3, **kw, **GAS)
         op.set_mesh(m)
         op.set_state(U)
@@ -339,4 +340,49 @@ def test_adjoint_api_errors():
         op.lserk_adjoint(U)
     with pytest.raises(GalxError, match="bad step"):
         op.forward_ckpt(0.0, 1e-3, 0, 1)
+    op.close()
+
+
+def test_mixed_context_vjp_equals_fp64():
+    """A mixed-precision context runs the same FP64 VJP kernels on FP64 trace records of its state: bitwise the
+    FP64 context's VJP at the same U."""
+    say("case: vjp mixed context vs fp64 context, (3,2,2) N=3 curved rotated")
+    m, U = noisy_case((3, 2, 2), 3, seed=21)
+    lam = np.asfortranarray(np.random.default_rng(4).standard_normal(U.shape))
+    res = []
+    for prec in ("fp64", "mixed"):
+        op = dgmod().DGOperator(3, form="split", riemann="lf", precision=prec, **GAS)
+        op.set_mesh(m)
+        op.set_state(U)
+        res.append(op.rhs_vjp(lam))
+        op.close()
+    assert np.array_equal(res[0], res[1])
+
+
+def test_mixed_forward_lserk_adjoint():
+    """Checkpointed adjoint along a mixed-precision forward (FP32 fluxes, FP64 adjoint accumulation): bitwise
+    independent of the checkpoint spacing, and within 1e-5 relative (L2) of the FP64 gradient, the size of the
+    mixed trajectory's own deviation."""
+    say("case: lserk adjoint along the mixed forward, (3,3,2) N=3, 12 steps")
+    m, U0 = lserk_case()
+    ref = gpu_op(m)
+    ref.set_state(U0)
+    dt, _ = ref.CalcTimeStep()
+    ref.forward_ckpt(0.0, dt, 12, 5)
+    g64 = ref.lserk_adjoint(kinetic_energy_grad(m, ref.get_state()))
+    ref.close()
+    op = dgmod().DGOperator(3, form="split", riemann="lf", precision="mixed", **GAS)
+    op.set_mesh(m)
+    res = []
+    for every in (1, 5, 12):
+        op.set_state(U0)
+        op.forward_ckpt(0.0, dt, 12, every)
+        UN = op.get_state()
+        res.append(op.lserk_adjoint(kinetic_energy_grad(m, UN)))
+        assert np.array_equal(op.get_state(), UN)
+    for r in res[1:]:
+        assert np.array_equal(r, res[0])
+    rel = np.linalg.norm((res[0] - g64).ravel()) / np.linalg.norm(g64.ravel())
+    say(f"  mixed-forward gradient vs FP64: rel L2 {rel:.2e}")
+    assert 0 < rel <= 1e-5
     op.close()
