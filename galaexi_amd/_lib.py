@@ -22,6 +22,7 @@ EXPORTS = (
     "galx_surf_int", "galx_timers_enable", "galx_timer_read", "galx_timers_reset", "galx_memory_bytes",
     "galx_rhs_vjp", "galx_rhs_vjp_device", "galx_lserk_forward_ckpt", "galx_lserk_adjoint",
     "galx_lserk_adjoint_device", "galx_lserk_stage", "galx_get_gradients", "galx_get_gradients_device",
+    "galx_step_vjp", "galx_step_vjp_device",
 )
 
 
@@ -81,6 +82,8 @@ def lib():
         "galx_lserk_forward_ckpt": ([vp, d, d, i, i], i),
         "galx_lserk_adjoint": ([vp, dp], i),
         "galx_lserk_adjoint_device": ([vp, vp], i),
+        "galx_step_vjp": ([vp, C.c_double, C.c_double, dp], i),
+        "galx_step_vjp_device": ([vp, C.c_double, C.c_double, vp], i),
         "galx_lserk_stage": ([vp, d, d, i], i),
         "galx_get_gradients": ([vp, dp, dp, dp], i),
         "galx_get_gradients_device": ([vp, vp, vp, vp], i),

@@ -210,12 +210,20 @@ __device__ __forceinline__ void lift_face_tasks(int e, int tid, const Mesh& mesh
 }
 
 template <int N>
-__device__ __forceinline__ bool lift_face_node_ubar(int e, int tid, const double* __restrict__ U, const GasD& gd,
-                                                    const double* s_f, double* Ub)
+__device__ __forceinline__ bool on_any_face(int tid)
 {
-    constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+    constexpr int n = N + 1, n2 = n * n;
     const int i = tid % n, j = (tid / n) % n, k = tid / n2;
-    if (i != 0 && i != N && j != 0 && j != N && k != 0 && k != N) return false;
+    return i == 0 || i == N || j == 0 || j == N || k == 0 || k == N;
+}
+
+// the conservative contribution at face node tid, from the state c(5) of that node
+template <int N>
+__device__ __forceinline__ void lift_face_node_ubar_at(int tid, const double* c, const GasD& gd, const double* s_f,
+                                                       double* Ub)
+{
+    constexpr int n = N + 1, n2 = n * n;
+    const int i = tid % n, j = (tid / n) % n, k = tid / n2;
     double qb[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int ls = 1; ls <= 6; ls++) {
@@ -224,10 +232,6 @@ __device__ __forceinline__ bool lift_face_node_ubar(int e, int tid, const double
 #pragma unroll
         for (int q = 0; q < 4; q++) qb[q] += s_f[(ls - 1) * 4 * n2 + q * n2 + ab];
     }
-    const double* u = U + (long)e * 5 * n3 + tid;
-    double c[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++) c[v] = u[v * n3];
     PrimBar pb;
     prim_bar_zero(pb);
     pb.u = qb[0];
@@ -235,12 +239,26 @@ __device__ __forceinline__ bool lift_face_node_ubar(int e, int tid, const double
     pb.w = qb[2];
     pb.T = qb[3];
     prim_bar_to_cons(c, pb, gd.kappa, gd.R, Ub);
+}
+
+template <int N>
+__device__ __forceinline__ bool lift_face_node_ubar(int e, int tid, const double* __restrict__ U, const GasD& gd,
+                                                    const double* s_f, double* Ub)
+{
+    constexpr int n3 = Dm<N>::n3;
+    if (!on_any_face<N>(tid)) return false;
+    const double* u = U + (long)e * 5 * n3 + tid;
+    double c[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) c[v] = u[v * n3];
+    lift_face_node_ubar_at<N>(tid, c, gd, s_f, Ub);
     return true;
 }
 
 // ---------------------------------------------------------------------------------------------------------
 // k_adj_pre: lam = kbar + bdt*Ubar (kbar updated in place when Ubar is given), mu traces
-//   mutr [side][2 slots][5][n2] = L_Hat(locSide) * (-sJ * lam) at the face nodes of the writer.
+//   mutr [side][2 slots][5][n2] = L_Hat(locSide) * (-sJ * lam) at the face nodes of the writer (the same
+//   product, in the same order, as k_adj_main forms for its own side).
 //   Uprev (optional, with Ubar): first completes the VJP of the stage before (in reverse order), evaluated at
 //   Uprev, with its pending neighbour lifting term from nghtr (the work of k_adj_face, fused).
 // ---------------------------------------------------------------------------------------------------------
@@ -251,37 +269,46 @@ __global__ void __launch_bounds__(Dm<N>::block)
 {
     using D = Dm<N>;
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
-    __shared__ double s_l[5 * n3];
+    __shared__ double s_l[5 * n3];  // -sJ * lam
     __shared__ double s_f[6 * 4 * n2];
     const int e = blockIdx.x, tid = threadIdx.x;
+    const bool active = tid < n3;
+    const bool face = active && Uprev && Ubar && on_any_face<N>(tid);
+    // the node's own loads first: none of them waits for the face phase
+    double kb5[5] = {0, 0, 0, 0, 0}, u5[5] = {0, 0, 0, 0, 0}, up[5] = {1, 0, 0, 0, 1}, msJ = 0.0;
+    const long off = (long)e * 5 * n3 + tid;
+    if (active) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) kb5[v] = kbar[off + v * n3];
+        if (Ubar) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) u5[v] = Ubar[off + v * n3];
+        }
+        if (face) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) up[v] = Uprev[off + v * n3];
+        }
+        msJ = -mesh.geo[(long)e * 10 * n3 + 9 * n3 + tid];
+    }
     if (Uprev) {
         lift_face_tasks<N>(e, tid, mesh, nghtr, s_f);
         __syncthreads();
     }
-    if (tid < n3) {
-        double* kb = kbar + (long)e * 5 * n3 + tid;
-        if (Ubar) {
-            double* ub = Ubar + (long)e * 5 * n3 + tid;
-            double u5[5];
-#pragma unroll
-            for (int v = 0; v < 5; v++) u5[v] = ub[v * n3];
+    if (active) {
+        if (face) {
             double add[5];
-            if (Uprev && lift_face_node_ubar<N>(e, tid, Uprev, gd, s_f, add)) {
-#pragma unroll
-                for (int v = 0; v < 5; v++) {
-                    u5[v] += add[v];
-                    ub[v * n3] = u5[v];
-                }
-            }
+            lift_face_node_ubar_at<N>(tid, up, gd, s_f, add);
 #pragma unroll
             for (int v = 0; v < 5; v++) {
-                double l = kb[v * n3] + bdt * u5[v];
-                kb[v * n3] = l;
-                s_l[v * n3 + tid] = l;
+                u5[v] += add[v];
+                Ubar[off + v * n3] = u5[v];
             }
-        } else {
+        }
 #pragma unroll
-            for (int v = 0; v < 5; v++) s_l[v * n3 + tid] = kb[v * n3];
+        for (int v = 0; v < 5; v++) {
+            double l = Ubar ? kb5[v] + bdt * u5[v] : kb5[v];
+            if (Ubar) kbar[off + v * n3] = l;
+            s_l[v * n3 + tid] = msJ * l;
         }
     }
     __syncthreads();
@@ -292,10 +319,10 @@ __global__ void __launch_bounds__(Dm<N>::block)
         long side = c >> 3;
         int flip = c & 7;
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
-        double f = -(lhat_of(ls, LHm, LHp) * mesh.geo[(long)e * 10 * n3 + 9 * n3 + node]);
+        const double LH = lhat_of(ls, LHm, LHp);
         double* o = mutr + ((side * 2 + (flip ? 1 : 0)) * 5) * n2 + pq;
 #pragma unroll
-        for (int v = 0; v < 5; v++) o[v * n2] = f * s_l[v * n3 + node];
+        for (int v = 0; v < 5; v++) o[v * n2] = LH * s_l[v * n3 + node];
     }
 }
 
@@ -312,11 +339,11 @@ template <int N>
 struct AdjLds {
     static constexpr int n2 = Dm<N>::n2, n3 = Dm<N>::n3;
     static constexpr int w_a = 9 * n3;               // split adjoint partner data: rho,u,v,w,p,H + metric
-    static constexpr int w_b = 4 * n3 + 6 * 7 * n2;  // forward lifting; the lifting adjoint contraction (4 n3)
+    static constexpr int w_b = 4 * Lay<N>::n3 + 6 * 7 * n2;  // forward lifting (padded layout); also >= 4 n3
     static constexpr int w_d = 6 * 12 * n2;          // face adjoint results
     static constexpr int w_ab = w_a > w_b ? w_a : w_b;
     static constexpr int w = w_ab > w_d ? w_ab : w_d;
-    static constexpr int bytes = 8 * (w + 5 * n3 + 6 * 6 * n2 + 3 * (N + 1) * (N + 1));
+    static constexpr int bytes = 8 * (w + 5 * n3 + 6 * 6 * n2 + 3 * Lay<N>::np * (N + 1));
 };
 
 // waves per SIMD that let two workgroups share a CU when their LDS fits (160 KB per CU); otherwise the
@@ -335,6 +362,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
 {
     using D = Dm<N>;
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
+    constexpr int np = Lay<N>::np, n3p = Lay<N>::n3;  // matrices and the lifting input: padded layout
     __shared__ double s_w[AdjLds<N>::w];
     __shared__ double s_mu[5 * n3];
     __shared__ double s_tau[6 * 6 * n2];  // own stress (txx,tyy,tzz,txy,txz,tyz) at the face nodes
@@ -388,7 +416,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
                 // arguments, so the same formula gives 2 DVS(i,i) dF#/dU_i^T mu_i
                 for (int l = 0; l <= N; l++) {
                     int o = base + l * stride;
-                    double dli = M.DVS[l + n * idx], dil = M.DVS[idx + n * l];
+                    double dli = M.DVS[l + np * idx], dil = M.DVS[idx + np * l];
                     double wv[5];
 #pragma unroll
                     for (int v = 0; v < 5; v++) wv[v] = dli * s_mu[v * n3 + tid] + dil * s_mu[v * n3 + o];
@@ -416,14 +444,15 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
     visc_grad_zero(gb);
     {
         if (active) {
-            s_w[tid] = pr[1];
-            s_w[n3 + tid] = pr[2];
-            s_w[2 * n3 + tid] = pr[3];
-            s_w[3 * n3 + tid] = pr[5];
+            const int pt = Lay<N>::at(tid);  // the lifting reads its input in the padded node layout
+            s_w[pt] = pr[1];
+            s_w[n3p + pt] = pr[2];
+            s_w[2 * n3p + pt] = pr[3];
+            s_w[3 * n3p + pt] = pr[5];
         }
         __syncthreads();
         double g[12];
-        lift_gradient<N, double, double>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3, gas, LHm, LHp, g);
+        lift_gradient<N, double, double>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3p, gas, LHm, LHp, g);
         if (active) {
             Stress<double> st;
             stress<double>(pr, g, gas, st);
@@ -444,7 +473,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
                 int base = tid - idx * stride;
                 double fb[5] = {0, 0, 0, 0, 0};
                 for (int m = 0; m <= N; m++) {
-                    double dh = M.DhT[idx + n * m];
+                    double dh = M.DhT[idx + np * m];
 #pragma unroll
                     for (int v = 1; v < 5; v++) fb[v] += dh * s_mu[v * n3 + base + m * stride];
                 }
@@ -577,7 +606,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
             int stride = d == 0 ? 1 : (d == 1 ? n : n2);
             int base = tid - idx * stride;
             for (int m = 0; m <= N; m++) {
-                double dd = M.DT[idx + n * m];  // D(m, idx)
+                double dd = M.DT[idx + np * m];  // D(m, idx)
 #pragma unroll
                 for (int q = 0; q < 4; q++) qb[q] += dd * s_a[q * n3 + base + m * stride];
             }

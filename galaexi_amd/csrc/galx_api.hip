@@ -1181,6 +1181,38 @@ int galx_lserk_adjoint_device(galx_ctx* c, double* dUbar)
     return GALX_OK;
 }
 
+// one recorded step, swept back, then the state restored to the step's input (the recorded checkpoint)
+static int step_vjp_restore(galx_ctx* c)
+{
+    size_t ub = 5L * c->nElems * c->n3 * sizeof(double);
+    HIPCHK(hipMemcpyAsync(c->dU, c->ckpt[0], ub, hipMemcpyDeviceToDevice, c->stream));
+    c->cur = 0;
+    c->ks.trace(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[0], gas_of(c));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->trajValid = false;
+    c->utValid = false;
+    return GALX_OK;
+}
+
+int galx_step_vjp_device(galx_ctx* c, double t, double dt, double* dUbar)
+{
+    if (!c || !dUbar) return fail(GALX_E_ARG, "null argument");
+    int rc = galx_lserk_forward_ckpt(c, t, dt, 1, 1);
+    if (rc) return rc;
+    if ((rc = galx_lserk_adjoint_device(c, dUbar))) return rc;
+    return step_vjp_restore(c);
+}
+
+int galx_step_vjp(galx_ctx* c, double t, double dt, double* Ubar)
+{
+    if (!c || !Ubar) return fail(GALX_E_ARG, "null argument");
+    int rc = galx_lserk_forward_ckpt(c, t, dt, 1, 1);
+    if (rc) return rc;
+    if ((rc = galx_lserk_adjoint(c, Ubar))) return rc;
+    return step_vjp_restore(c);
+}
+
 int galx_lserk_adjoint(galx_ctx* c, double* Ubar)
 {
     int rc = adj_check(c);

@@ -28,6 +28,9 @@
 #ifndef GALX_MIXED_WAVES
 #define GALX_MIXED_WAVES 6
 #endif
+#ifndef GALX_LDS_PAD
+#define GALX_LDS_PAD 0
+#endif
 
 namespace galx {
 
@@ -36,6 +39,16 @@ struct Dm {
     static constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, nf = 6 * n2;
     static constexpr int block = n3 <= 64 ? 64 : ((n3 + 63) / 64) * 64;
     static constexpr int ft = (nf + block - 1) / block;
+};
+
+// LDS layout of the per-node arrays read along element lines (lifting, split and viscous volume terms) and of
+// the D-matrices: node (i,j,k) at i + np*(j + n*k) with row stride np = n + GALX_LDS_PAD.  A padded stride (n+1)
+// spreads the rows of an i-line read over more LDS banks, but measured slower at N=7 (k_rhs FP64 1.99 vs 1.84 ms,
+// mixed 1.31 vs 1.26 ms), so the default is the dense layout (GALX_LDS_PAD 0).
+template <int N>
+struct Lay {
+    static constexpr int n = N + 1, np = n + GALX_LDS_PAD, n3 = np * n * n;  // n3: padded array length
+    __device__ static __forceinline__ int at(int node) { return node + GALX_LDS_PAD * (node / n); }
 };
 
 // volume node of face node (a,b) of local side ls (GL trace); ls numbering of src/flexi.h:98-103
@@ -91,23 +104,26 @@ struct Stage {
     int mode;  // 0: write Ut; 1: LSERK register update
 };
 
-// matrices staged in LDS: D_T (transpose of D), D_Hat_T, DVolSurf, all (0:N,0:N) column-major
+// matrices staged in LDS: D_T (transpose of D), D_Hat_T, DVolSurf, all (0:N,0:N) column-major with leading
+// dimension Lay<N>::np: entry (a,b) at a + np*b
 template <int N, typename T>
 struct Mats {
-    T DT[(N + 1) * (N + 1)];
-    T DhT[(N + 1) * (N + 1)];
-    T DVS[(N + 1) * (N + 1)];
+    static constexpr int ld = Lay<N>::np;
+    T DT[ld * (N + 1)];
+    T DhT[ld * (N + 1)];
+    T DVS[ld * (N + 1)];
 };
 
 template <int N, typename T>
 __device__ __forceinline__ void load_mats(Mats<N, T>& M, const T* __restrict__ g, int tid, int nthreads)
 {
-    constexpr int nn = (N + 1) * (N + 1);
+    constexpr int n = N + 1, nn = n * n, ld = Lay<N>::np;
     for (int x = tid; x < 3 * nn; x += nthreads) {
         T v = g[x];
-        if (x < nn) M.DT[x] = v;
-        else if (x < 2 * nn) M.DhT[x - nn] = v;
-        else M.DVS[x - 2 * nn] = v;
+        int r = x % nn, o = (r % n) + ld * (r / n);
+        if (x < nn) M.DT[o] = v;
+        else if (x < 2 * nn) M.DhT[o] = v;
+        else M.DVS[o] = v;
     }
 }
 
@@ -116,7 +132,8 @@ __device__ __forceinline__ void load_mats(Mats<N, T>& M, const T* __restrict__ g
 // lifting_volint.t90:262-328, Lifting_FillFlux lifting_fillflux.t90:39-93, SurfInt_Device_Single_Flux with
 // doApplyJacobian surfint.t90:591-725).  Lifted variables (u,v,w,T); the density gradient of the reference's
 // nVarLifting=5 never enters the fluxes and is not formed.
-//   s_lift  : T[4][n3]   own (u,v,w,T), filled by the caller before the call (synchronised)
+//   s_lift  : T[4][Lay<N>::n3]   own (u,v,w,T) in the padded node layout, filled by the caller before the call
+//             (synchronised)
 //   s_lflux : T[6][7][n2] scratch
 //   s_nb    : T[6][5][n2] (optional) receives the neighbour trace records of the six faces, for later face tasks
 // returns g[12] = grad(u,v,w,T) in x, then y, then z for node tid (valid if tid < n3).
@@ -151,7 +168,7 @@ __device__ __forceinline__ void lift_face_fluxes(int e, int tid, const Mesh& mes
         T* o = s_lflux + (ls - 1) * 7 * n2 + ab;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            T own = s_lift[q * n3 + node];
+            T own = s_lift[q * Lay<N>::n3 + Lay<N>::at(node)];
             T Lm = flip ? Lnb[q] : own, Lsl = flip ? own : Lnb[q];
             o[q * n2] = T(0.5) * se * (T(-1) * Lm + Lsl);
         }
@@ -168,16 +185,17 @@ __device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh,
                                               const T* s_lift, const T* s_lflux, T LHm, T LHp, T* g)
 {
     constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+    constexpr int np = Lay<N>::np, n3p = Lay<N>::n3;
     const int i = node % n, j = (node / n) % n, k = node / n2;
     T gxi[4] = {0, 0, 0, 0}, get[4] = {0, 0, 0, 0}, gze[4] = {0, 0, 0, 0};
 #pragma unroll GALX_UNROLL_L
     for (int l = 0; l <= N; l++) {
-        T di = M.DT[l + n * i], dj = M.DT[l + n * j], dk = M.DT[l + n * k];
+        T di = M.DT[l + np * i], dj = M.DT[l + np * j], dk = M.DT[l + np * k];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            gxi[q] = gxi[q] + di * s_lift[q * n3 + l + n * (j + n * k)];
-            get[q] = get[q] + dj * s_lift[q * n3 + i + n * (l + n * k)];
-            gze[q] = gze[q] + dk * s_lift[q * n3 + i + n * (j + n * l)];
+            gxi[q] = gxi[q] + di * s_lift[q * n3p + l + np * (j + n * k)];
+            get[q] = get[q] + dj * s_lift[q * n3p + i + np * (l + n * k)];
+            gze[q] = gze[q] + dk * s_lift[q * n3p + i + np * (j + n * l)];
         }
     }
     const double* ge = mesh.geo + (long)e * 10 * n3 + node;
@@ -302,26 +320,28 @@ __global__ void __launch_bounds__(Dm<N>::block)
 {
     using D = Dm<N>;
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
-    __shared__ T s_buf[4 * n3 + 6 * 12 * n2];  // own (u,v,w,T) | lifting fluxes, then own face gradients
+    constexpr int n3p = Lay<N>::n3;
+    __shared__ T s_buf[4 * n3p + 6 * 12 * n2];  // own (u,v,w,T) | lifting fluxes, then own face gradients
     __shared__ Mats<N, T> M;
     const int e = blockIdx.x, tid = threadIdx.x;
     T* s_lift = s_buf;
-    T* s_lflux = s_buf + 4 * n3;
+    T* s_lflux = s_buf + 4 * n3p;
     load_mats<N, T>(M, mats, tid, D::block);
     const Gas<T> gas = make_gas<T>(gd);
     T uc[5], pr[6], H;
     if (tid < n3) {
         load_state<N, T>(e, tid, U, gd, uc, pr, H);
-        s_lift[tid] = pr[1];
-        s_lift[n3 + tid] = pr[2];
-        s_lift[2 * n3 + tid] = pr[3];
-        s_lift[3 * n3 + tid] = pr[5];
+        const int pt = Lay<N>::at(tid);
+        s_lift[pt] = pr[1];
+        s_lift[n3p + pt] = pr[2];
+        s_lift[2 * n3p + pt] = pr[3];
+        s_lift[3 * n3p + pt] = pr[5];
     }
     __syncthreads();
     T g[12];
     lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_lift, s_lflux, gas, LHm, LHp, g);
     __syncthreads();
-    T* s_g = s_buf + 4 * n3;  // [6][12][n2]
+    T* s_g = s_buf + 4 * n3p;  // [6][12][n2]
     if (tid < n3) {
         int i = tid % n, j = (tid / n) % n, k = tid / n2;
 #pragma unroll
@@ -343,7 +363,8 @@ __global__ void __launch_bounds__(Dm<N>::block)
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
         const double* fg = mesh.fgeo + side * 10 * n2 + pq;
         T nv[3] = {T(fg[0]), T(fg[n2]), T(fg[2 * n2])};
-        T P[4] = {T(0), s_lift[node], s_lift[n3 + node], s_lift[2 * n3 + node]};
+        const int pn = Lay<N>::at(node);
+        T P[4] = {T(0), s_lift[pn], s_lift[n3p + pn], s_lift[2 * n3p + pn]};
         T G[12];
         const T* sg = s_g + (ls - 1) * 12 * n2 + ab;
 #pragma unroll
@@ -367,7 +388,8 @@ __global__ void __launch_bounds__(Dm<N>::block)
 {
     using D = Dm<N>;
     constexpr int n2 = D::n2, n3 = D::n3;
-    __shared__ T s_buf[4 * n3 + 6 * 7 * n2];
+    constexpr int n3p = Lay<N>::n3;
+    __shared__ T s_buf[4 * n3p + 6 * 7 * n2];
     __shared__ Mats<N, T> M;
     const int e = blockIdx.x, tid = threadIdx.x;
     load_mats<N, T>(M, mats, tid, D::block);
@@ -375,14 +397,15 @@ __global__ void __launch_bounds__(Dm<N>::block)
     T uc[5], pr[6], H;
     if (tid < n3) {
         load_state<N, T>(e, tid, U, gd, uc, pr, H);
-        s_buf[tid] = pr[1];
-        s_buf[n3 + tid] = pr[2];
-        s_buf[2 * n3 + tid] = pr[3];
-        s_buf[3 * n3 + tid] = pr[5];
+        const int pt = Lay<N>::at(tid);
+        s_buf[pt] = pr[1];
+        s_buf[n3p + pt] = pr[2];
+        s_buf[2 * n3p + pt] = pr[3];
+        s_buf[3 * n3p + pt] = pr[5];
     }
     __syncthreads();
     T g[12];
-    lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_buf, s_buf + 4 * n3, gas, LHm, LHp, g);
+    lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_buf, s_buf + 4 * n3p, gas, LHm, LHp, g);
     if (tid < n3) {
         double* o = gvol + (long)e * 12 * n3 + tid;
 #pragma unroll
@@ -445,8 +468,9 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 {
     using D = Dm<N>;
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
-    constexpr int w_a = 4 * n3 + 6 * 7 * n2;          // lifting
-    constexpr int w_c = SPLIT ? 13 * n3 : 5 * n3;     // volume fluxes / split-form partner data
+    constexpr int np = Lay<N>::np, n3p = Lay<N>::n3;  // padded node layout of the line-read arrays
+    constexpr int w_a = 4 * n3p + 6 * 7 * n2;         // lifting
+    constexpr int w_c = SPLIT ? 13 * n3p : 5 * n3p;   // volume fluxes / split-form partner data
     constexpr int w_d = 6 * 5 * n2;                   // face results, new traces
     constexpr int w_ac = w_a > w_c ? w_a : w_c;
     constexpr int lds_w = w_ac > w_d ? w_ac : w_d;
@@ -456,21 +480,22 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     const int e = blockIdx.x, tid = threadIdx.x;
     const bool active = tid < n3;
     const int i = tid % n, j = (tid / n) % n, k = tid / n2;
+    const int pt = Lay<N>::at(tid);
     load_mats<N, T>(M, mats, tid, D::block);
     const Gas<T> gas = make_gas<T>(gd);
     T uc[5], pr[6], Hme = T(0);
     if (active) {
         load_state<N, T>(e, tid, U, gd, uc, pr, Hme);
-        s_w[tid] = pr[1];
-        s_w[n3 + tid] = pr[2];
-        s_w[2 * n3 + tid] = pr[3];
-        s_w[3 * n3 + tid] = pr[5];
+        s_w[pt] = pr[1];
+        s_w[n3p + pt] = pr[2];
+        s_w[2 * n3p + pt] = pr[3];
+        s_w[3 * n3p + pt] = pr[5];
     }
     __syncthreads();
     Stress<T> s;
     {
         T g[12];
-        lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3, gas, LHm, LHp, g, s_nb);
+        lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3p, gas, LHm, LHp, g, s_nb);
         if (active) stress(pr, g, gas, s);
     }
     __syncthreads();
@@ -479,7 +504,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     const double* ge = mesh.geo + (long)e * 10 * n3 + tid;
     if (!SPLIT) {
         // ---- full transformed fluxes, weak-form D_Hat_T contraction (VolInt_weakForm) ----
-        T* s_f = s_w;  // [5][n3]
+        T* s_f = s_w;  // [5][n3p]
 #pragma unroll 1
         for (int d = 0; d < 3; d++) {
             if (active) {
@@ -487,18 +512,18 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                 T f[5];
                 transformed_flux<T, true>(uc, pr, s, m, gas.hshift, f);
 #pragma unroll
-                for (int v = 0; v < 5; v++) s_f[v * n3 + tid] = f[v];
+                for (int v = 0; v < 5; v++) s_f[v * n3p + pt] = f[v];
             }
             __syncthreads();
             if (active) {
                 int idx = d == 0 ? i : (d == 1 ? j : k);
-                int stride = d == 0 ? 1 : (d == 1 ? n : n2);
-                int base = tid - idx * stride;
+                int stride = d == 0 ? 1 : (d == 1 ? np : np * n);
+                int base = pt - idx * stride;
 #pragma unroll GALX_UNROLL_L
                 for (int l = 0; l <= N; l++) {
-                    T dh = M.DhT[l + n * idx];
+                    T dh = M.DhT[l + np * idx];
 #pragma unroll
-                    for (int v = 0; v < 5; v++) ut[v] = ut[v] + dh * s_f[v * n3 + base + l * stride];
+                    for (int v = 0; v < 5; v++) ut[v] = ut[v] + dh * s_f[v * n3p + base + l * stride];
                 }
             }
             __syncthreads();
@@ -507,16 +532,16 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         // ---- per direction: viscous contravariant flux with the weak-form D_Hat_T contraction
         // (VolInt_weakForm_Visc) and the split-form flux differencing (VolInt_splitForm_Kernel); the direction's
         // metric is read once for both ----
-        T* s_v = s_w;           // [6][n3] rho,u,v,w,p,H
-        T* s_m = s_w + 6 * n3;  // [3][n3] metric of the current direction
-        T* s_f = s_w + 9 * n3;  // [4][n3] viscous contravariant flux (momentum, energy) of the current direction
+        T* s_v = s_w;            // [6][n3p] rho,u,v,w,p,H
+        T* s_m = s_w + 6 * n3p;  // [3][n3p] metric of the current direction
+        T* s_f = s_w + 9 * n3p;  // [4][n3p] viscous contravariant flux (momentum, energy) of the current direction
         if (active) {
-            s_v[tid] = pr[0];
-            s_v[n3 + tid] = pr[1];
-            s_v[2 * n3 + tid] = pr[2];
-            s_v[3 * n3 + tid] = pr[3];
-            s_v[4 * n3 + tid] = pr[4];
-            s_v[5 * n3 + tid] = Hme;
+            s_v[pt] = pr[0];
+            s_v[n3p + pt] = pr[1];
+            s_v[2 * n3p + pt] = pr[2];
+            s_v[3 * n3p + pt] = pr[3];
+            s_v[4 * n3p + pt] = pr[4];
+            s_v[5 * n3p + pt] = Hme;
         }
         T tmp[5] = {0, 0, 0, 0, 0};
 #pragma unroll 1
@@ -526,24 +551,24 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                 mme[0] = T(ge[(3 * d) * n3]);
                 mme[1] = T(ge[(3 * d + 1) * n3]);
                 mme[2] = T(ge[(3 * d + 2) * n3]);
-                s_m[tid] = mme[0];
-                s_m[n3 + tid] = mme[1];
-                s_m[2 * n3 + tid] = mme[2];
+                s_m[pt] = mme[0];
+                s_m[n3p + pt] = mme[1];
+                s_m[2 * n3p + pt] = mme[2];
                 T f[5];
                 transformed_flux<T, false>(uc, pr, s, mme, gas.hshift, f);
 #pragma unroll
-                for (int v = 1; v < 5; v++) s_f[(v - 1) * n3 + tid] = f[v];
+                for (int v = 1; v < 5; v++) s_f[(v - 1) * n3p + pt] = f[v];
             }
             __syncthreads();
             if (active) {
                 int idx = d == 0 ? i : (d == 1 ? j : k);
-                int stride = d == 0 ? 1 : (d == 1 ? n : n2);
-                int base = tid - idx * stride;
+                int stride = d == 0 ? 1 : (d == 1 ? np : np * n);
+                int base = pt - idx * stride;
 #pragma unroll GALX_UNROLL_L
                 for (int l = 0; l <= N; l++) {
-                    T dh = M.DhT[l + n * idx];
+                    T dh = M.DhT[l + np * idx];
 #pragma unroll
-                    for (int v = 1; v < 5; v++) ut[v] = ut[v] + dh * s_f[(v - 1) * n3 + base + l * stride];
+                    for (int v = 1; v < 5; v++) ut[v] = ut[v] + dh * s_f[(v - 1) * n3p + base + l * stride];
                 }
                 // all l, the diagonal included as in VolInt_splitForm_Kernel (volint.f90:307-317): DVolSurf(i,i)
                 // vanishes for Gauss-Lobatto up to rounding (dg.f90:229-233), and a uniform loop keeps the
@@ -551,12 +576,12 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll GALX_UNROLL_SPLIT
                 for (int l = 0; l <= N; l++) {
                     int o = base + l * stride;
-                    T Mo[3] = {s_m[o], s_m[n3 + o], s_m[2 * n3 + o]};
+                    T Mo[3] = {s_m[o], s_m[n3p + o], s_m[2 * n3p + o]};
                     T F[5];
-                    split_volume_flux_pi<T>(pr[0], pr[1], pr[2], pr[3], pr[4], Hme, s_v[o], s_v[n3 + o],
-                                            s_v[2 * n3 + o], s_v[3 * n3 + o], s_v[4 * n3 + o], s_v[5 * n3 + o],
+                    split_volume_flux_pi<T>(pr[0], pr[1], pr[2], pr[3], pr[4], Hme, s_v[o], s_v[n3p + o],
+                                            s_v[2 * n3p + o], s_v[3 * n3p + o], s_v[4 * n3p + o], s_v[5 * n3p + o],
                                             mme, Mo, F);
-                    T dv = M.DVS[l + n * idx];
+                    T dv = M.DVS[l + np * idx];
 #pragma unroll
                     for (int v = 0; v < 5; v++) tmp[v] = tmp[v] + dv * F[v];
                 }

@@ -161,6 +161,16 @@ class DGOperator:
     def lserk_adjoint_device(self, ubar_ptr):
         check(lib().galx_lserk_adjoint_device(self._h, C.c_void_p(ubar_ptr)), "galx_lserk_adjoint_device")
 
+    def step_vjp(self, t, dt, Ubar_next):
+        """dJ/dU_n from dJ/dU_{n+1} for one LSERK step of size dt at the current state (which is kept)."""
+        Ubar = _F(Ubar_next).copy(order="F")
+        assert Ubar.shape == self._shape()
+        check(lib().galx_step_vjp(self._h, float(t), float(dt), _dptr(Ubar)), "galx_step_vjp")
+        return Ubar
+
+    def step_vjp_device(self, t, dt, ubar_ptr):
+        check(lib().galx_step_vjp_device(self._h, float(t), float(dt), C.c_void_p(ubar_ptr)), "galx_step_vjp_device")
+
     def timers(self, on=True):
         check(lib().galx_timers_enable(self._h, int(on)))
 

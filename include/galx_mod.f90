@@ -93,6 +93,12 @@ INTERFACE
     TYPE(C_PTR),VALUE             :: ctx
     REAL(C_DOUBLE),INTENT(INOUT)  :: Ubar(*)
   END FUNCTION
+  INTEGER(C_INT) FUNCTION galx_step_vjp(ctx,t,dt,Ubar) BIND(C,NAME='galx_step_vjp')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR),VALUE             :: ctx
+    REAL(C_DOUBLE),VALUE          :: t, dt
+    REAL(C_DOUBLE),INTENT(INOUT)  :: Ubar(*)
+  END FUNCTION
   TYPE(C_PTR) FUNCTION galx_last_error() BIND(C,NAME='galx_last_error')
     IMPORT :: C_PTR
   END FUNCTION

@@ -386,3 +386,42 @@ def test_mixed_forward_lserk_adjoint():
     say(f"  mixed-forward gradient vs FP64: rel L2 {rel:.2e}")
     assert 0 < rel <= 1e-5
     op.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_step_vjp(precision):
+    """galx_step_vjp: bitwise the adjoint of a recorded one-step trajectory, the context kept at its state, and
+    (FP64) against central FD of one LSERK step: J(U) = w . step(U), best over eps within 1e-6."""
+    say(f"case: step vjp {precision}, (3,3,2) N=3")
+    m, U0 = lserk_case()
+    rng = np.random.default_rng(17)
+    w = np.asfortranarray(rng.standard_normal(U0.shape))
+    op = dgmod().DGOperator(3, form="split", riemann="lf", precision=precision, **GAS)
+    op.set_mesh(m)
+    op.set_state(U0)
+    dt, _ = op.CalcTimeStep()
+    g = op.step_vjp(0.0, dt, w)
+    assert np.array_equal(op.get_state(), U0)
+    ref = dgmod().DGOperator(3, form="split", riemann="lf", precision=precision, **GAS)
+    ref.set_mesh(m)
+    ref.set_state(U0)
+    ref.forward_ckpt(0.0, dt, 1, 1)
+    assert np.array_equal(g, ref.lserk_adjoint(w))
+    # the context steps on from the kept state exactly like a fresh one
+    op.TimeStepByLSERKW2(0.0, dt, 1)
+    ref.set_state(U0)
+    ref.TimeStepByLSERKW2(0.0, dt, 1)
+    assert np.array_equal(op.get_state(), ref.get_state())
+    if precision == "fp64":
+        def J(X):
+            ref.set_state(np.asfortranarray(X))
+            ref.TimeStepByLSERKW2(0.0, dt, 1)
+            return float(np.sum(w * ref.get_state()))
+
+        v = direction(U0, rng)
+        exact = float(np.sum(g * v))
+        best, errs = best_fd_error(J, U0, v, exact, [1e-3, 1e-4, 1e-5, 1e-6])
+        say(f"  dJ.v={exact:.6e} rel errs {['%.1e' % e for e in errs]}")
+        assert best <= 1e-6
+    op.close()
+    ref.close()

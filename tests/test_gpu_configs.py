@@ -1,0 +1,91 @@
+"""Two survey configurations on the GPU at their own sizes.
+
+C1: isentropic vortex (ExactFunc CASE(7), exactfunc.f90:482-503) with seeded centre in [-1,1]^2 and amplitude in
+[0.5,1], base flow rho=1, p=1/kappa, u=(0.5,0,0) (Ma 0.5), on the periodic box [-5,5]^3 with 8^3 elements, N=3
+Gauss-Lobatto, weak form, Roe, BR1, mu0=1e-4, R=1, Pr=0.72, CFLScale=DFLScale=0.9, dt from CALCTIMESTEP at t=0
+held fixed, 100 LSERK steps.  The whole run against the FP64 oracle's run: relative L2 <= 1e-12 after 500 operator
+evaluations (measured on MI355X: 1.0e-15).
+
+C2: TGV Re1600 Ma0.1 on [0,2pi]^3 with 32^3 elements, N=7, split PI + LF, BR1, seeded 1e-3 velocity noise.  Too
+large for the oracle at test speed; the size-independent property is discrete conservation.  On a periodic mesh
+sum_i w_i J_i Ut_i = 0 for every conservative variable, so after 20 steps the integrals of rho, rho*u, rho*v, rho*w
+and E stay constant to rounding.  Bar against each one's natural scale (int rho, int rho|u|, int E): 1e-13
+(measured on MI355X: 2.6e-14 for rho, 1.3e-15 for the momenta, 4.9e-16 for E; the oracle on the 8^3 mesh: below
+3e-15).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from galaexi_amd import mesh as M
+from galaexi_amd import testcases as T
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def c1_case(seed=1):
+    rng = np.random.default_rng(seed)
+    center = (rng.uniform(-1.0, 1.0), rng.uniform(-1.0, 1.0), 0.0)
+    amp = rng.uniform(0.5, 1.0)
+    m = M.box_mesh((8, 8, 8), 3, lo=(-5.0, -5.0, -5.0), hi=(5.0, 5.0, 5.0))
+    U = T.shu_vortex(m.Elem_xGP, center=center, amplitude=amp, halfwidth=1.0, axis=(0.0, 0.0, 1.0),
+                     rho0=1.0, vel0=(0.5, 0.0, 0.0), kappa=1.4)
+    return m, U, center, amp
+
+
+def test_c1_isentropic_vortex_100_steps_vs_oracle():
+    from galaexi_amd import dg
+
+    m, U0, center, amp = c1_case()
+    say(f"case: C1 vortex 8^3 N=3 weak Roe, centre {center[:2]}, amplitude {amp:.3f}, 100 steps")
+    gas = dict(kappa=1.4, R=1.0, Pr=0.72, mu0=1e-4)
+    op = dg.DGOperator(3, form="weak", riemann="roe", precision="fp64", CFLScale=0.9, DFLScale=0.9, **gas)
+    op.set_mesh(m)
+    op.set_state(U0)
+    dt, err = op.CalcTimeStep()
+    assert err == 0
+    ref = O.Operator(m, split=False, riemann=O.RIEMANN_ROE, **gas)
+    dt_ref, err_ref, _, _ = ref.calc_dt(U0, 0.9, 0.9)
+    assert err_ref == 0 and abs(dt - dt_ref) <= 1e-14 * dt_ref
+    op.TimeStepByLSERKW2(0.0, dt, nSteps=100)
+    Ug = op.get_state()
+    Ur = np.asfortranarray(U0.copy())
+    K = np.zeros_like(Ur, order="F")
+    for s in range(100):
+        ref.lserk_step(Ur, K, s * dt_ref, dt_ref)
+    rel = np.linalg.norm((Ug - Ur).ravel()) / np.linalg.norm(Ur.ravel())
+    moved = np.linalg.norm((Ur - U0).ravel()) / np.linalg.norm(U0.ravel())
+    say(f"  t_end = {100 * dt:.4f}: GPU vs oracle rel L2 {rel:.2e} (the state itself changed by {moved:.2e})")
+    assert rel <= 1e-12
+    op.close()
+
+
+def test_c2_tgv_full_size_conservation():
+    from galaexi_amd import dg
+
+    say("case: C2 TGV 32^3 N=7 split LF, 20 steps, conservation of the five integrals")
+    m = M.box_mesh((32, 32, 32), 7, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3)
+    U0 = T.taylor_green(m.Elem_xGP, Ma=0.1, noise=1e-3, seed=1600)
+    op = dg.DGOperator(7, form="split", riemann="lf", precision="fp64", kappa=1.4, R=71.42857, Pr=0.72,
+                       mu0=1.0 / 1600.0, CFLScale=0.5, DFLScale=0.5)
+    op.set_mesh(m)
+    op.set_state(U0)
+    dt, err = op.CalcTimeStep()
+    assert err == 0
+    op.TimeStepByLSERKW2(0.0, dt, nSteps=20)
+    U1 = op.get_state()
+    op.close()
+    w = O.dg_basis(7)["wGP"]
+    W = w[:, None, None, None] * w[None, :, None, None] * w[None, None, :, None] / m.sJ
+    I0 = np.array([np.sum(W * U0[v]) for v in range(5)])
+    I1 = np.array([np.sum(W * U1[v]) for v in range(5)])
+    speed = np.sqrt(U0[1] ** 2 + U0[2] ** 2 + U0[3] ** 2)  # rho |u|
+    scale = np.array([np.sum(W * U0[0]), *(3 * [np.sum(W * speed)]), np.sum(W * U0[4])])
+    drift = np.abs(I1 - I0) / scale
+    say(f"  drift of the integrals over 20 steps (relative to their scales): {['%.1e' % d for d in drift]}")
+    assert np.all(drift <= 1e-13)
+    assert np.abs(U1 - U0).max() > 1e-6  # the state did evolve
