@@ -343,7 +343,7 @@ struct AdjLds {
     static constexpr int w_d = 6 * 12 * n2;          // face adjoint results
     static constexpr int w_ab = w_a > w_b ? w_a : w_b;
     static constexpr int w = w_ab > w_d ? w_ab : w_d;
-    static constexpr int bytes = 8 * (w + 5 * n3 + 6 * 6 * n2 + 3 * Lay<N>::np * (N + 1));
+    static constexpr int bytes = 8 * (w + 5 * n3 + 6 * 6 * n2 + 3 * Lay<N>::np * (N + 1));  // + Mats
 };
 
 // waves per SIMD that let two workgroups share a CU when their LDS fits (160 KB per CU); otherwise the
@@ -366,11 +366,11 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
     __shared__ double s_w[AdjLds<N>::w];
     __shared__ double s_mu[5 * n3];
     __shared__ double s_tau[6 * 6 * n2];  // own stress (txx,tyy,tzz,txy,txz,tyz) at the face nodes
-    __shared__ Mats<N, double> M;
+    __shared__ Mats<N, double, false> M;  // column-major: the adjoint contractions read (idx, m)
     const int e = blockIdx.x, tid = threadIdx.x;
     const bool active = tid < n3;
     const int i = tid % n, j = (tid / n) % n, k = tid / n2;
-    load_mats<N, double>(M, mats, tid, D::block);
+    load_mats<N, double, false>(M, mats, tid, D::block);
     const Gas<double> gas = make_gas<double>(gd);
     double pr[6] = {1, 0, 0, 0, 1, 1};
     const double* ge = mesh.geo + (long)e * 10 * n3 + tid;
@@ -416,7 +416,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
                 // arguments, so the same formula gives 2 DVS(i,i) dF#/dU_i^T mu_i
                 for (int l = 0; l <= N; l++) {
                     int o = base + l * stride;
-                    double dli = M.DVS[l + np * idx], dil = M.DVS[idx + np * l];
+                    double dli = M.dvs(l, idx), dil = M.dvs(idx, l);
                     double wv[5];
 #pragma unroll
                     for (int v = 0; v < 5; v++) wv[v] = dli * s_mu[v * n3 + tid] + dil * s_mu[v * n3 + o];
@@ -473,7 +473,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
                 int base = tid - idx * stride;
                 double fb[5] = {0, 0, 0, 0, 0};
                 for (int m = 0; m <= N; m++) {
-                    double dh = M.DhT[idx + np * m];
+                    double dh = M.dht(idx, m);
 #pragma unroll
                     for (int v = 1; v < 5; v++) fb[v] += dh * s_mu[v * n3 + base + m * stride];
                 }
@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
             int stride = d == 0 ? 1 : (d == 1 ? n : n2);
             int base = tid - idx * stride;
             for (int m = 0; m <= N; m++) {
-                double dd = M.DT[idx + np * m];  // D(m, idx)
+                double dd = M.dt(idx, m);  // D(m, idx)
 #pragma unroll
                 for (int q = 0; q < 4; q++) qb[q] += dd * s_a[q * n3 + base + m * stride];
             }

@@ -31,6 +31,12 @@
 #ifndef GALX_LDS_PAD
 #define GALX_LDS_PAD 0
 #endif
+#ifndef GALX_MATS_T
+#define GALX_MATS_T 1
+#endif
+#ifndef GALX_LDS_SWZ
+#define GALX_LDS_SWZ 0
+#endif
 
 namespace galx {
 
@@ -45,10 +51,26 @@ struct Dm {
 // the D-matrices: node (i,j,k) at i + np*(j + n*k) with row stride np = n + GALX_LDS_PAD.  A padded stride (n+1)
 // spreads the rows of an i-line read over more LDS banks, but measured slower at N=7 (k_rhs FP64 1.99 vs 1.84 ms,
 // mixed 1.31 vs 1.26 ms), so the default is the dense layout (GALX_LDS_PAD 0).
+// With GALX_LDS_SWZ (n a power of two) the i index is stored XOR-swizzled by j, (i ^ j) + np*(j + n*k): the
+// lanes of an i-line read (same l, different j) then hit different banks without padding.  Measured at N=7:
+// k_rhs 1.86 vs 1.82 ms in FP64 and 1.29 vs 1.26 ms mixed, so it is off by default like the padding.
 template <int N>
 struct Lay {
     static constexpr int n = N + 1, np = n + GALX_LDS_PAD, n3 = np * n * n;  // n3: padded array length
-    __device__ static __forceinline__ int at(int node) { return node + GALX_LDS_PAD * (node / n); }
+    static constexpr bool swz = GALX_LDS_SWZ != 0 && (n & (n - 1)) == 0;
+    __device__ static __forceinline__ int idx(int i, int j, int k) { return (swz ? (i ^ j) : i) + np * (j + n * k); }
+    __device__ static __forceinline__ int at(int node) { return idx(node % n, (node / n) % n, node / (n * n)); }
+    // node l of the direction-d line through (i,j,k): C + (swizzled l) + s*l, with C, x, a, s fixed per line
+    struct Line {
+        int C, x, a, s;
+        __device__ __forceinline__ int operator()(int l) const { return C + (a ? (l ^ x) : 0) + s * l; }
+    };
+    __device__ static __forceinline__ Line line(int d, int i, int j, int k)
+    {
+        if (d == 0) return swz ? Line{np * (j + n * k), j, 1, 0} : Line{np * (j + n * k), 0, 0, 1};
+        if (d == 1) return swz ? Line{np * n * k, i, 1, np} : Line{i + np * n * k, 0, 0, np};
+        return Line{(swz ? (i ^ j) : i) + np * j, 0, 0, np * n};
+    }
 };
 
 // volume node of face node (a,b) of local side ls (GL trace); ls numbering of src/flexi.h:98-103
@@ -104,23 +126,31 @@ struct Stage {
     int mode;  // 0: write Ut; 1: LSERK register update
 };
 
-// matrices staged in LDS: D_T (transpose of D), D_Hat_T, DVolSurf, all (0:N,0:N) column-major with leading
-// dimension Lay<N>::np: entry (a,b) at a + np*b
-template <int N, typename T>
+// matrices staged in LDS: D_T (transpose of D), D_Hat_T, DVolSurf, all (0:N,0:N); dt(a,b), dht(a,b), dvs(a,b)
+// return entry (a,b) of the column-major matrix.  The forward kernels read (l, idx), idx the lane's own node
+// index along a line and l common to the wave: stored transposed (TR, idx fastest), the lanes of a wave read
+// consecutive entries instead of entries n apart, and k_rhs/k_lift run 2.6 %/1.5 % faster in FP64.  The adjoint
+// reads (idx, m) in its contractions and keeps the column-major storage (TR false).  Leading dimension
+// Lay<N>::np.
+template <int N, typename T, bool TR = (GALX_MATS_T != 0)>
 struct Mats {
     static constexpr int ld = Lay<N>::np;
     T DT[ld * (N + 1)];
     T DhT[ld * (N + 1)];
     T DVS[ld * (N + 1)];
+    __device__ static __forceinline__ int off(int a, int b) { return TR ? b + ld * a : a + ld * b; }
+    __device__ __forceinline__ T dt(int a, int b) const { return DT[off(a, b)]; }
+    __device__ __forceinline__ T dht(int a, int b) const { return DhT[off(a, b)]; }
+    __device__ __forceinline__ T dvs(int a, int b) const { return DVS[off(a, b)]; }
 };
 
-template <int N, typename T>
-__device__ __forceinline__ void load_mats(Mats<N, T>& M, const T* __restrict__ g, int tid, int nthreads)
+template <int N, typename T, bool TR>
+__device__ __forceinline__ void load_mats(Mats<N, T, TR>& M, const T* __restrict__ g, int tid, int nthreads)
 {
-    constexpr int n = N + 1, nn = n * n, ld = Lay<N>::np;
+    constexpr int n = N + 1, nn = n * n;
     for (int x = tid; x < 3 * nn; x += nthreads) {
         T v = g[x];
-        int r = x % nn, o = (r % n) + ld * (r / n);
+        int r = x % nn, o = Mats<N, T, TR>::off(r % n, r / n);
         if (x < nn) M.DT[o] = v;
         else if (x < 2 * nn) M.DhT[o] = v;
         else M.DVS[o] = v;
@@ -180,8 +210,8 @@ __device__ __forceinline__ void lift_face_fluxes(int e, int tid, const Mesh& mes
 
 // BR1 gradient g[12] = grad(u,v,w,T) (x, then y, then z) at volume node `node` of element e, from s_lift and the
 // lifting fluxes of lift_face_fluxes (the node half of lift_gradient; any thread may evaluate any node)
-template <int N, typename T>
-__device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh, const Mats<N, T>& M,
+template <int N, typename T, bool TR>
+__device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh, const Mats<N, T, TR>& M,
                                               const T* s_lift, const T* s_lflux, T LHm, T LHp, T* g)
 {
     constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
@@ -190,12 +220,12 @@ __device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh,
     T gxi[4] = {0, 0, 0, 0}, get[4] = {0, 0, 0, 0}, gze[4] = {0, 0, 0, 0};
 #pragma unroll GALX_UNROLL_L
     for (int l = 0; l <= N; l++) {
-        T di = M.DT[l + np * i], dj = M.DT[l + np * j], dk = M.DT[l + np * k];
+        T di = M.dt(l, i), dj = M.dt(l, j), dk = M.dt(l, k);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            gxi[q] = gxi[q] + di * s_lift[q * n3p + l + np * (j + n * k)];
-            get[q] = get[q] + dj * s_lift[q * n3p + i + np * (l + n * k)];
-            gze[q] = gze[q] + dk * s_lift[q * n3p + i + np * (j + n * l)];
+            gxi[q] = gxi[q] + di * s_lift[q * n3p + Lay<N>::idx(l, j, k)];
+            get[q] = get[q] + dj * s_lift[q * n3p + Lay<N>::idx(i, l, k)];
+            gze[q] = gze[q] + dk * s_lift[q * n3p + Lay<N>::idx(i, j, l)];
         }
     }
     const double* ge = mesh.geo + (long)e * 10 * n3 + node;
@@ -232,14 +262,14 @@ __device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh,
     }
 }
 
-template <int N, typename T, typename TT>
+template <int N, typename T, typename TT, bool TR>
 __device__ __forceinline__ void lift_gradient(int e, int tid, const Mesh& mesh, const TT* __restrict__ utr,
-                                              const Mats<N, T>& M, const T* s_lift, T* s_lflux, const Gas<T>& gas,
-                                              T LHm, T LHp, T* g, T* s_nb = nullptr)
+                                              const Mats<N, T, TR>& M, const T* s_lift, T* s_lflux,
+                                              const Gas<T>& gas, T LHm, T LHp, T* g, T* s_nb = nullptr)
 {
     lift_face_fluxes<N, T, TT>(e, tid, mesh, utr, s_lift, s_lflux, gas, s_nb);
     __syncthreads();
-    if (tid < Dm<N>::n3) node_gradient<N, T>(e, tid, mesh, M, s_lift, s_lflux, LHm, LHp, g);
+    if (tid < Dm<N>::n3) node_gradient<N, T, TR>(e, tid, mesh, M, s_lift, s_lflux, LHm, LHp, g);
 }
 
 // volume state of node tid: ConsToPrim (eos.f90:212-239) in FP64, then shifted by the pressure offset and
@@ -517,13 +547,13 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
             __syncthreads();
             if (active) {
                 int idx = d == 0 ? i : (d == 1 ? j : k);
-                int stride = d == 0 ? 1 : (d == 1 ? np : np * n);
-                int base = pt - idx * stride;
+                const auto ln = Lay<N>::line(d, i, j, k);
 #pragma unroll GALX_UNROLL_L
                 for (int l = 0; l <= N; l++) {
-                    T dh = M.DhT[l + np * idx];
+                    T dh = M.dht(l, idx);
+                    const int o = ln(l);
 #pragma unroll
-                    for (int v = 0; v < 5; v++) ut[v] = ut[v] + dh * s_f[v * n3p + base + l * stride];
+                    for (int v = 0; v < 5; v++) ut[v] = ut[v] + dh * s_f[v * n3p + o];
                 }
             }
             __syncthreads();
@@ -562,26 +592,26 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
             __syncthreads();
             if (active) {
                 int idx = d == 0 ? i : (d == 1 ? j : k);
-                int stride = d == 0 ? 1 : (d == 1 ? np : np * n);
-                int base = pt - idx * stride;
+                const auto ln = Lay<N>::line(d, i, j, k);
 #pragma unroll GALX_UNROLL_L
                 for (int l = 0; l <= N; l++) {
-                    T dh = M.DhT[l + np * idx];
+                    T dh = M.dht(l, idx);
+                    const int o = ln(l);
 #pragma unroll
-                    for (int v = 1; v < 5; v++) ut[v] = ut[v] + dh * s_f[(v - 1) * n3p + base + l * stride];
+                    for (int v = 1; v < 5; v++) ut[v] = ut[v] + dh * s_f[(v - 1) * n3p + o];
                 }
                 // all l, the diagonal included as in VolInt_splitForm_Kernel (volint.f90:307-317): DVolSurf(i,i)
                 // vanishes for Gauss-Lobatto up to rounding (dg.f90:229-233), and a uniform loop keeps the
                 // wave free of per-lane branches
 #pragma unroll GALX_UNROLL_SPLIT
                 for (int l = 0; l <= N; l++) {
-                    int o = base + l * stride;
+                    const int o = ln(l);
                     T Mo[3] = {s_m[o], s_m[n3p + o], s_m[2 * n3p + o]};
                     T F[5];
                     split_volume_flux_pi<T>(pr[0], pr[1], pr[2], pr[3], pr[4], Hme, s_v[o], s_v[n3p + o],
                                             s_v[2 * n3p + o], s_v[3 * n3p + o], s_v[4 * n3p + o], s_v[5 * n3p + o],
                                             mme, Mo, F);
-                    T dv = M.DVS[l + np * idx];
+                    T dv = M.dvs(l, idx);
 #pragma unroll
                     for (int v = 0; v < 5; v++) tmp[v] = tmp[v] + dv * F[v];
                 }
