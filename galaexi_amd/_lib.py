@@ -22,7 +22,7 @@ EXPORTS = (
     "galx_surf_int", "galx_timers_enable", "galx_timer_read", "galx_timers_reset", "galx_memory_bytes",
     "galx_rhs_vjp", "galx_rhs_vjp_device", "galx_lserk_forward_ckpt", "galx_lserk_adjoint",
     "galx_lserk_adjoint_device", "galx_lserk_stage", "galx_get_gradients", "galx_get_gradients_device",
-    "galx_step_vjp", "galx_step_vjp_device",
+    "galx_step_vjp", "galx_step_vjp_device", "galx_set_source",
 )
 
 
@@ -65,6 +65,7 @@ def lib():
         "galx_get_state": ([vp, dp], i),
         "galx_set_state_device": ([vp, vp], i),
         "galx_get_state_device": ([vp, vp], i),
+        "galx_set_source": ([vp, i, dp, dp], i),
         "galx_rhs": ([vp, d], i),
         "galx_get_ut": ([vp, dp], i),
         "galx_get_ut_device": ([vp, vp], i),

@@ -160,6 +160,38 @@ __global__ void k_grad_to_ref(long nDOF, int n3, const double* __restrict__ g, d
     }
 }
 
+// manufactured-solution source of the synthetic test cases, CalcSource_CUDA_Kernel
+// (src/equations/navierstokes/idealgas/exactfunc.f90:946-1111).  Every case it implements (ExactFunc 4, 41, 42,
+// 43) has the form  S_v = c[v][0] cos(phi) + c[v][1] sin(2 phi) + c[v][2] sin(phi),  phi = omega*xs - a*t, with xs
+// the node's coordinate combination (x+y+z for 4; x, y or z for 41, 42, 43) and the table c built on the host.
+// S is a physical-space rate: the reference adds S/sJ before scaling Ut by sJ, which is S added after it.
+// mode 0 (after the RHS kernel wrote Ut):  X = Ut,  X += S.
+// mode 1 (before an LSERK stage whose RHS kernel then runs with mRKA = 1):  X = K,  X = mRKA*K + S.
+struct SourceCoefs {
+    double c[5][3];
+    double omega, a;
+};
+
+__global__ void __launch_bounds__(256) k_source(long nDOF, int n3, const double* __restrict__ xs,
+                                                double* __restrict__ X, SourceCoefs sc, double t, double mRKA,
+                                                int mode)
+{
+    long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nDOF) return;
+    long e = x / n3;
+    int node = (int)(x - e * n3);
+    double sn, cs;
+    sincos(sc.omega * xs[x] - sc.a * t, &sn, &cs);
+    double sn2 = 2.0 * sn * cs;
+    double* p = X + e * 5 * n3 + node;
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        double S = sc.c[v][0] * cs + sc.c[v][1] * sn2 + sc.c[v][2] * sn;
+        double o = p[v * n3];
+        p[v * n3] = mode == 0 ? o + S : o * mRKA + S;
+    }
+}
+
 // sum of the pressure over all nodes (element planes), for the pressure offset of the mixed form
 __global__ void __launch_bounds__(256) k_pressure_sum(long nDOF, int n3, double kappa, const double* __restrict__ U,
                                                       double* __restrict__ out)
@@ -377,6 +409,11 @@ struct galx_ctx {
     int trajSteps = 0, trajEvery = 0;
     bool trajValid = false;
     double *dGrad = nullptr, *dGradRef = nullptr;  // BR1 gradients for analysis (planes, reference layout)
+    // manufactured-solution source (ExactFunc 4, 41, 42, 43): coefficient table and each node's coordinate
+    // combination [e][n3]
+    int srcFunc = 0;
+    SourceCoefs src{};
+    double* dSrcX = nullptr;
     long long bytes = 0;
     bool timing = false;
     std::map<std::string, Timer> timers;
@@ -533,7 +570,8 @@ int galx_destroy(galx_ctx* c)
     void* ps[] = {c->dU, c->dK, c->dUt, c->dScratch, c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dUtr[0], c->dUtr[1],
                   c->dVtr, c->dMats, c->dDtc, c->dDtv, c->dDtOut, c->dErr};
     for (void* p : ps) dfree(p);
-    void* pa[] = {c->dKbar, c->dUbar, c->dMutr, c->dNghtr, c->dUfinal, c->dGrad, c->dGradRef, c->dMats64, c->dTr64};
+    void* pa[] = {c->dKbar,    c->dUbar,   c->dMutr,    c->dNghtr, c->dUfinal,
+                  c->dGrad,    c->dGradRef, c->dMats64, c->dTr64,  c->dSrcX};
     for (void* p : pa) dfree(p);
     for (double* p : c->ckpt) dfree(p);
     for (double* p : c->stages) dfree(p);
@@ -690,6 +728,70 @@ int galx_set_state_device(galx_ctx* c, const double* dU)
     return after_state(c);
 }
 
+// CalcSource's terms (src/equations/navierstokes/idealgas/exactfunc.f90:946-1111) as one coefficient table per
+// case, with the constants of that kernel (Frequency 1, Amplitude 0.1, C 2); InitExactFunc switches the source on
+// for IniExactFunc 4, 41, 42, 43 and off otherwise (exactfunc.f90:131-139).  The viscous terms are those of the
+// parabolic build; mu0 = 0 drops them.
+int galx_set_source(galx_ctx* c, int exactFunc, const double* advVel, const double* Elem_xGP)
+{
+    if (!c) return fail(GALX_E_ARG, "null context");
+    if (!c->meshSet) return fail(GALX_E_STATE, "set_mesh first");
+    if (exactFunc != 4 && exactFunc != 41 && exactFunc != 42 && exactFunc != 43) {
+        c->srcFunc = 0;
+        return GALX_OK;
+    }
+    if (!advVel || !Elem_xGP) return fail(GALX_E_ARG, "null argument");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    const double pi = std::acos(-1.0), frequency = 1.0, A = 0.1, C = 2.0;
+    const double kappa = c->cfg.kappa, mu0 = c->cfg.mu0, Pr = c->cfg.Pr;
+    const int dir = exactFunc == 4 ? 0 : exactFunc - 41;  // the AdvVel component that sets the wave speed
+    SourceCoefs sc{};
+    sc.omega = pi * frequency;
+    sc.a = advVel[dir] * 2.0 * pi;
+    const double w = sc.omega, a = sc.a;
+    if (exactFunc == 4) {  // oblique wave in x+y+z, velocity (1,1,1)
+        double tmp[6] = {-a + 3.0 * w,
+                         -a + 0.5 * w * (1.0 + kappa * 5.0),
+                         A * w * (kappa - 1.0),
+                         0.5 * ((9.0 + kappa * 15.0) * w - 8.0 * a),
+                         A * (3.0 * w * kappa - a),
+                         3.0 * mu0 * kappa * w * w / Pr};
+        for (double& x : tmp) x *= A;
+        sc.c[0][0] = tmp[0];
+        for (int v = 1; v <= 3; v++) {
+            sc.c[v][0] = tmp[1];
+            sc.c[v][1] = tmp[2];
+        }
+        sc.c[4][0] = tmp[3];
+        sc.c[4][1] = tmp[4];
+        sc.c[4][2] = tmp[5];
+    } else {  // plane wave along x, y or z, unit velocity along it
+        const int mom = 1 + dir;
+        sc.c[0][0] = -A * a + A * w;
+        sc.c[mom][0] = -A * a + 2.0 * A * w * kappa * C - 0.5 * A * w * kappa + 1.5 * A * w - 2.0 * A * w * C;
+        sc.c[mom][1] = -A * A * w + A * A * w * kappa;
+        sc.c[4][0] = 0.5 * (-4.0 * A * a * C + 4.0 * A * w * kappa * C - A * w * kappa + A * w);
+        sc.c[4][1] = -A * A * a + A * A * w * kappa;
+        sc.c[4][2] = mu0 * kappa * A * w * w / Pr;
+    }
+    // each node's coordinate combination, element planes [e][n3] (x+y+z summed left to right as the reference's
+    // SUM(Elem_xGP(1:3,i)))
+    const long nDOF = (long)c->nElems * c->n3;
+    std::vector<double> xs(nDOF);
+    for (long i = 0; i < nDOF; i++) {
+        const double* x = Elem_xGP + 3 * i;
+        xs[i] = exactFunc == 4 ? (x[0] + x[1]) + x[2] : x[dir];
+    }
+    int rc;
+    if ((rc = dalloc(c, &c->dSrcX, nDOF * sizeof(double)))) return rc;
+    HIPCHK(hipMemcpyAsync(c->dSrcX, xs.data(), nDOF * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->src = sc;
+    c->srcFunc = exactFunc;
+    c->utValid = false;
+    return GALX_OK;
+}
+
 int galx_set_state(galx_ctx* c, const double* U)
 {
     if (!c || !U) return fail(GALX_E_ARG, "null argument");
@@ -740,21 +842,40 @@ static void launch_lift(galx_ctx* c, const double* U = nullptr, const void* utr 
                1.0 / B.w[c->N]);
 }
 
-// one RHS pass on the state U (default: the context state) with its traces utr, the updated state to Uout
-// (default: in place) and its traces to utr_next (defaults: the double-buffered context traces)
-static void launch_rhs(galx_ctx* c, Stage st, const double* U = nullptr, double* Uout = nullptr,
+// the manufactured source at time t into Ut (mode 0) or into the LSERK register (mode 1, K = mRKA*K + S)
+static void launch_source(galx_ctx* c, double t, int mode, double mRKA)
+{
+    TimerScope ts(c, "source");
+    long nDOF = (long)c->nElems * c->n3;
+    k_source<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dSrcX,
+                                                                              mode == 0 ? c->dUt : c->dK, c->src,
+                                                                              t, mRKA, mode);
+}
+
+// one RHS pass at time t (the stage time; it enters only through the manufactured source) on the state U
+// (default: the context state) with its traces utr, the updated state to Uout (default: in place) and its
+// traces to utr_next (defaults: the double-buffered context traces).  With a source, an LSERK stage first folds
+// it into the register (K = mRKA*K + S) and the RHS kernel then adds its own rate with mRKA = 1; a plain RHS
+// evaluation adds it to Ut afterwards.
+static void launch_rhs(galx_ctx* c, Stage st, double t, const double* U = nullptr, double* Uout = nullptr,
                        const void* utr = nullptr, void* utr_next = nullptr)
 {
-    TimerScope ts(c, "rhs");
-    const Basis& B = c->basis;
-    c->ks.rhs(c->stream, c->nElems, U ? U : c->dU, Uout ? Uout : c->dU, mesh_of(c), utr ? utr : c->dUtr[c->cur],
-              c->dVtr, utr_next ? utr_next : c->dUtr[c->cur ^ 1], c->dUt, c->dK, c->dMats, gas_of(c),
-              c->cfg.riemann, 1.0 / B.w[0], 1.0 / B.w[c->N], st);
+    if (c->srcFunc && st.mode == 1) {
+        launch_source(c, t, 1, st.mRKA);
+        st.mRKA = 1.0;
+    }
+    {
+        TimerScope ts(c, "rhs");
+        const Basis& B = c->basis;
+        c->ks.rhs(c->stream, c->nElems, U ? U : c->dU, Uout ? Uout : c->dU, mesh_of(c),
+                  utr ? utr : c->dUtr[c->cur], c->dVtr, utr_next ? utr_next : c->dUtr[c->cur ^ 1], c->dUt, c->dK,
+                  c->dMats, gas_of(c), c->cfg.riemann, 1.0 / B.w[0], 1.0 / B.w[c->N], st);
+    }
+    if (c->srcFunc && st.mode == 0) launch_source(c, t, 0, 0.0);
 }
 
 int galx_rhs(galx_ctx* c, double t)
 {
-    (void)t;
     if (!c) return fail(GALX_E_ARG, "null context");
     if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
     HIPCHK(hipSetDevice(c->cfg.device));
@@ -762,7 +883,7 @@ int galx_rhs(galx_ctx* c, double t)
     if ((rc = dalloc(c, &c->dUt, 5L * c->nElems * c->n3 * sizeof(double)))) return rc;
     launch_lift(c);
     HIPCHK(hipGetLastError());
-    launch_rhs(c, Stage{0.0, 0.0, 0});
+    launch_rhs(c, Stage{0.0, 0.0, 0}, t);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->utValid = true;
@@ -802,14 +923,22 @@ static const double RKA[5] = {0.0, 567301805773.0 / 1357537059087.0, 24042679903
 static const double RKB[5] = {1432997174477.0 / 9575080441755.0, 5161836677717.0 / 13612068292357.0,
                               1720146321549.0 / 2090206949498.0, 3134564353537.0 / 4481467310338.0,
                               2277821191437.0 / 14882151754819.0};
+static const double RKC[5] = {0.0, 1432997174477.0 / 9575080441755.0, 2526269341429.0 / 6820363962896.0,
+                              2006345519317.0 / 3224310063776.0, 2802321613138.0 / 2924317926251.0};
 
-static int lserk_enqueue(galx_ctx* c, double dt, int nSteps)
+// time of step iStep of a run that starts at t0, and of its stages (src/timedisc/timestep.f90:87-90); the forward
+// run and the adjoint's recompute of it use the same expressions, so they see the same stage times
+static double step_time(double t0, double dt, int iStep) { return t0 + iStep * dt; }
+static double stage_time(double t, double dt, int stage) { return stage == 0 ? t : t + RKC[stage] * dt; }
+
+// nSteps LSERK steps, the first of them step iStep0 of a run that starts at t0
+static int lserk_enqueue(galx_ctx* c, double t0, double dt, int iStep0, int nSteps)
 {
     for (int s = 0; s < nSteps; s++)
         for (int stage = 0; stage < 5; stage++) {
             launch_lift(c);
             double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
-            launch_rhs(c, Stage{mRKA, RKB[stage] * dt, 1});
+            launch_rhs(c, Stage{mRKA, RKB[stage] * dt, 1}, stage_time(step_time(t0, dt, iStep0 + s), dt, stage));
             c->cur ^= 1;
         }
     HIPCHK(hipGetLastError());
@@ -819,12 +948,11 @@ static int lserk_enqueue(galx_ctx* c, double dt, int nSteps)
 
 int galx_lserk_steps(galx_ctx* c, double t, double dt, int nSteps)
 {
-    (void)t;
     if (!c) return fail(GALX_E_ARG, "null context");
     if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
     if (nSteps < 0 || !(dt == dt)) return fail(GALX_E_ARG, "bad step arguments");
     HIPCHK(hipSetDevice(c->cfg.device));
-    int rc = lserk_enqueue(c, dt, nSteps);
+    int rc = lserk_enqueue(c, t, dt, 0, nSteps);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     return GALX_OK;
@@ -834,7 +962,6 @@ int galx_lserk_step(galx_ctx* c, double t, double dt) { return galx_lserk_steps(
 
 int galx_lserk_stage(galx_ctx* c, double t, double dt, int iStage)
 {
-    (void)t;
     if (!c) return fail(GALX_E_ARG, "null context");
     if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
     if (iStage < 1 || iStage > 5 || !(dt == dt)) return fail(GALX_E_ARG, "bad stage arguments");
@@ -842,7 +969,7 @@ int galx_lserk_stage(galx_ctx* c, double t, double dt, int iStage)
     int stage = iStage - 1;
     launch_lift(c);
     double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
-    launch_rhs(c, Stage{mRKA, RKB[stage] * dt, 1});
+    launch_rhs(c, Stage{mRKA, RKB[stage] * dt, 1}, stage_time(t, dt, stage));
     c->cur ^= 1;
     HIPCHK(hipGetLastError());
     c->utValid = false;
@@ -1092,7 +1219,7 @@ int galx_lserk_forward_ckpt(galx_ctx* c, double t, double dt, int nSteps, int ck
     for (int seg = 0; seg < nSeg; seg++) {
         HIPCHK(hipMemcpyAsync(c->ckpt[seg], c->dU, ub, hipMemcpyDeviceToDevice, c->stream));
         int n = std::min(ckptEvery, nSteps - seg * ckptEvery);
-        if ((rc = lserk_enqueue(c, dt, n))) return rc;
+        if ((rc = lserk_enqueue(c, t, dt, seg * ckptEvery, n))) return rc;
     }
     HIPCHK(hipMemcpyAsync(c->dUfinal, c->dU, ub, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1135,7 +1262,8 @@ static int adj_reverse(galx_ctx* c)
                 void* tro = keep ? (last ? c->dUtr[0] : (void*)c->stageTr[idx + 1]) : c->dUtr[(idx + 1) & 1];
                 launch_lift(c, Ui, tri);
                 double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
-                launch_rhs(c, Stage{mRKA, RKB[stage] * c->trajDt, 1}, Ui, Uo, tri, tro);
+                const double ts = stage_time(step_time(c->trajT, c->trajDt, s0 + s), c->trajDt, stage);
+                launch_rhs(c, Stage{mRKA, RKB[stage] * c->trajDt, 1}, ts, Ui, Uo, tri, tro);
             }
         HIPCHK(hipGetLastError());
         // reverse

@@ -85,6 +85,18 @@ class DGOperator:
         assert U.shape == self._shape(), (U.shape, self._shape())
         check(lib().galx_set_state(self._h, _dptr(U)), "galx_set_state")
 
+    # InitExactFunc's doCalcSource + CalcSource (src/equations/navierstokes/idealgas/exactfunc.f90:131-139,915-1111)
+    def set_source(self, exactFunc, AdvVel=(0.0, 0.0, 0.0), Elem_xGP=None):
+        """Manufactured source of ExactFunc 4, 41, 42 or 43 on every later RHS evaluation, at the stage time;
+        any other exactFunc switches it off.  Elem_xGP(3,0:N,0:N,0:N,nElems): the node coordinates."""
+        adv = _F(np.asarray(AdvVel, dtype=np.float64).reshape(3))
+        if Elem_xGP is None:
+            x = np.zeros(1, order="F")
+        else:
+            x = _F(Elem_xGP)
+            assert x.shape == (3, self.n, self.n, self.n, self.nElems), x.shape
+        check(lib().galx_set_source(self._h, int(exactFunc), _dptr(adv), _dptr(x)), "galx_set_source")
+
     def get_state(self):
         U = np.zeros(self._shape(), order="F")
         check(lib().galx_get_state(self._h, _dptr(U)), "galx_get_state")

@@ -4,6 +4,8 @@ Shu vortex: ExactFunc CASE(7) (src/equations/navierstokes/idealgas/exactfunc.f90
 Taylor-Green vortex: ExactFuncTestcase (src/testcase/taylorgreenvortex/testcase.f90:220-262), p0 from
 src/testcase/taylorgreenvortex/testcase.f90:140-148.
 PrimToCons: src/equations/navierstokes/idealgas/eos.f90 (E = p/(kappa-1) + rho|u|^2/2).
+Synthetic waves with a manufactured source: ExactFunc CASE(4), CASE(41), CASE(42), CASE(43)
+(src/equations/navierstokes/idealgas/exactfunc.f90:326-420).
 """
 from __future__ import annotations
 
@@ -85,3 +87,29 @@ def smooth_random(x, lo, hi, seed=0, kappa=1.4, kmax=2):
     rho = 1.0 + 0.1 * fields[0]
     return prim_to_cons(rho, 0.3 * fields[1], 0.3 * fields[2], 0.3 * fields[3], (1.0 + 0.1 * fields[4]) / kappa,
                         kappa)
+
+
+SOURCE_FUNCS = (4, 41, 42, 43)
+
+
+def synthetic_wave(case, x, t=0.0, AdvVel=(0.3, 0.3, 0.3)):
+    """ExactFunc CASE(4) / CASE(41..43) (exactfunc.f90:326-420), 3D, Frequency 1, Amplitude 0.1.
+    rho = 2 + 0.1 sin(pi*s - a*t) with s = x+y+z (4) or x, y, z (41, 42, 43) and a = 2*pi*AdvVel(d); the momentum
+    equals rho in every direction of the wave (velocity 1 there, 0 elsewhere) and E = rho^2.
+    x(3, ...) -> U(5, ...)."""
+    if case not in SOURCE_FUNCS:
+        raise ValueError(f"exact function {case} is not one of the synthetic waves {SOURCE_FUNCS}")
+    omega, amp = np.pi * 1.0, 0.1
+    d = 0 if case == 4 else case - 41
+    a = AdvVel[d] * 2.0 * np.pi
+    s = (x[0] + x[1]) + x[2] if case == 4 else x[d]
+    rho = 2.0 + amp * np.sin(omega * s - a * t)
+    U = np.zeros((5,) + np.shape(rho), order="F")
+    U[0] = rho
+    if case == 4:
+        U[1] = U[2] = U[3] = rho
+    else:
+        U[1 + d] = rho
+    U[4] = rho * rho
+    return np.asfortranarray(U)
+

@@ -77,6 +77,12 @@ int galx_set_mesh(galx_ctx *ctx, int nElems, int nSides, const int *ElemToSide, 
                   const double *Metrics_hTilde, const double *sJ, const double *NormVec, const double *TangVec1,
                   const double *TangVec2, const double *SurfElem);
 
+/* CalcSource (src/equations/navierstokes/idealgas/exactfunc.f90:915-1111) as switched on by InitExactFunc
+ * (exactfunc.f90:131-139): for exactFunc 4, 41, 42, 43 every later RHS evaluation adds the manufactured source of
+ * that exact function at its stage time (timestep.f90:87-90); any other value switches it off.  AdvVel(3) and
+ * Elem_xGP(3,0:N,0:N,0:N,nElems) (the node coordinates, reference layout) are host arrays; needs the mesh. */
+int galx_set_source(galx_ctx *ctx, int exactFunc, const double *AdvVel, const double *Elem_xGP);
+
 /* d_U = U and U = d_U (src/dg/dg.f90:166-169, src/timedisc/timedisc_func.f90:348-349), host arrays. */
 int galx_set_state(galx_ctx *ctx, const double *U);
 int galx_get_state(galx_ctx *ctx, double *U);
@@ -84,8 +90,8 @@ int galx_get_state(galx_ctx *ctx, double *U);
 int galx_set_state_device(galx_ctx *ctx, const double *dU);
 int galx_get_state_device(galx_ctx *ctx, double *dU);
 
-/* DGTimeDerivative_weakForm(t) (src/dg/dg.f90:255-425): Ut from the current state.  t is accepted for
- * interface parity; the periodic operator has no explicit time dependence. */
+/* DGTimeDerivative_weakForm(t) (src/dg/dg.f90:255-425): Ut from the current state.  t enters only through the
+ * manufactured source (galx_set_source); without one the periodic operator has no explicit time dependence. */
 int galx_rhs(galx_ctx *ctx, double t);
 /* Ut of the last galx_rhs in the reference layout (host / device pointer). */
 int galx_get_ut(galx_ctx *ctx, double *Ut);

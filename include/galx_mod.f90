@@ -43,6 +43,12 @@ INTERFACE
     TYPE(C_PTR),VALUE          :: ctx
     REAL(C_DOUBLE),INTENT(OUT) :: U(*)
   END FUNCTION
+  INTEGER(C_INT) FUNCTION galx_set_source(ctx,exactFunc,AdvVel,Elem_xGP) BIND(C,NAME='galx_set_source')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR),VALUE         :: ctx
+    INTEGER(C_INT),VALUE      :: exactFunc
+    REAL(C_DOUBLE),INTENT(IN) :: AdvVel(3), Elem_xGP(*)
+  END FUNCTION
   INTEGER(C_INT) FUNCTION galx_rhs(ctx,t) BIND(C,NAME='galx_rhs')
     IMPORT :: C_INT, C_PTR, C_DOUBLE
     TYPE(C_PTR),VALUE    :: ctx

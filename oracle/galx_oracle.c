@@ -301,6 +301,10 @@ struct orc_ctx {
     double *UPrim, *U_master, *U_slave, *UPrim_master, *UPrim_slave;
     double *gradUx, *gradUy, *gradUz, *gxm, *gym, *gzm, *gxs, *gys, *gzs, *liftFlux;
     double *Flux_master, *Flux_slave, *f, *g, *h, *Ut_stage;
+    /* manufactured source: exact function (0: none), AdvVel, node coordinates Elem_xGP(3,nDOF) */
+    int src_func;
+    double adv[3];
+    double *Elem_xGP;
 };
 
 static void *xcalloc(long n, long sz) { return calloc((size_t)(n > 0 ? n : 1), (size_t)sz); }
@@ -346,7 +350,8 @@ void orc_destroy(orc_ctx *c)
     void *p[] = {c->ElemToSide, c->SideToElem, c->S2V2, c->S2V2_inv, c->Mf, c->Mg, c->Mh, c->sJ, c->NormVec,
                  c->TangVec1, c->TangVec2, c->SurfElem, c->UPrim, c->U_master, c->U_slave, c->UPrim_master,
                  c->UPrim_slave, c->gradUx, c->gradUy, c->gradUz, c->gxm, c->gym, c->gzm, c->gxs, c->gys,
-                 c->gzs, c->liftFlux, c->Flux_master, c->Flux_slave, c->f, c->g, c->h, c->Ut_stage};
+                 c->gzs, c->liftFlux, c->Flux_master, c->Flux_slave, c->f, c->g, c->h, c->Ut_stage,
+                 c->Elem_xGP};
     for (unsigned i = 0; i < sizeof(p) / sizeof(p[0]); i++) free(p[i]);
     free(c);
 }
@@ -757,8 +762,76 @@ static void lifting_br1(orc_ctx *c)
     prolong_gl(c, 5, c->gradUz, c->gzm, c->gzs);
 }
 
-/* DGTimeDerivative_weakForm (src/dg/dg.f90:255-425) */
-int orc_rhs(orc_ctx *c, const double *U, double *Ut)
+/* InitExactFunc's doCalcSource (src/equations/navierstokes/idealgas/exactfunc.f90:131-139): exact functions 4,
+ * 41, 42, 43 carry a source, any other switches it off. */
+int orc_set_source(orc_ctx *c, int exactFunc, const double *advVel, const double *Elem_xGP)
+{
+    long nDOF = (long)c->n * c->n * c->n * c->nElems;
+    if (exactFunc != 4 && exactFunc != 41 && exactFunc != 42 && exactFunc != 43) {
+        c->src_func = 0;
+        return 0;
+    }
+    if (!c->Elem_xGP) c->Elem_xGP = (double *)xcalloc(3 * nDOF, 8);
+    memcpy(c->Elem_xGP, Elem_xGP, (size_t)(3 * nDOF) * sizeof(double));
+    for (int d = 0; d < 3; d++) c->adv[d] = advVel[d];
+    c->src_func = exactFunc;
+    return 0;
+}
+
+/* Ut_src of CalcSource_CUDA_Kernel (exactfunc.f90:946-1111), parabolic 3D build, at one node x and time t:
+ * the physical-space source rate of the exact function. */
+static void source_point(const orc_ctx *c, const double *x, double t, double *S)
+{
+    const double Frequency = 1.0, Amplitude = 0.1, Cc = 2.0;
+    const double kappa = c->kappa, mu0 = c->mu0, Pr = c->Pr;
+    const double pi = acos(-1.0);
+    const double Omega = pi * Frequency;
+    for (int v = 0; v < 5; v++) S[v] = 0.0;
+    if (c->src_func == 4) { /* exactfunc.f90:991-1022 */
+        double a = c->adv[0] * 2.0 * pi, tmp[6];
+        tmp[0] = -a + 3.0 * Omega;
+        tmp[1] = -a + 0.5 * Omega * (1.0 + kappa * 5.0);
+        tmp[2] = Amplitude * Omega * (kappa - 1.0);
+        tmp[3] = 0.5 * ((9.0 + kappa * 15.0) * Omega - 8.0 * a);
+        tmp[4] = Amplitude * (3.0 * Omega * kappa - a);
+        tmp[5] = 3.0 * mu0 * kappa * Omega * Omega / Pr;
+        for (int q = 0; q < 6; q++) tmp[q] *= Amplitude;
+        double arg = Omega * (x[0] + x[1] + x[2]) - a * t;
+        double cs = cos(arg), sn = sin(arg), sn2 = 2.0 * sn * cs;
+        S[0] = tmp[0] * cs;
+        for (int v = 1; v <= 3; v++) S[v] = tmp[1] * cs + tmp[2] * sn2;
+        S[4] = tmp[3] * cs + tmp[4] * sn2 + tmp[5] * sn;
+    } else { /* 41, 42, 43: exactfunc.f90:1024-1101 */
+        int d = c->src_func - 41;
+        double A = Amplitude, a = c->adv[d] * 2.0 * pi, xd = x[d];
+        double cs = cos(Omega * xd - a * t), sn = sin(Omega * xd - a * t);
+        double sn2 = sin(2.0 * Omega * xd - 2.0 * a * t);
+        S[0] = (-A * a + A * Omega) * cs;
+        S[1 + d] = (-A * A * Omega + A * A * Omega * kappa) * sn2 +
+                   (-A * a + 2.0 * A * Omega * kappa * Cc - 0.5 * A * Omega * kappa + 1.5 * A * Omega -
+                    2.0 * A * Omega * Cc) * cs;
+        S[4] = mu0 * kappa * A * sn * Omega * Omega / Pr + (-A * A * a + A * A * Omega * kappa) * sn2 +
+               0.5 * (-4.0 * A * a * Cc + 4.0 * A * Omega * kappa * Cc - A * Omega * kappa + A * Omega) * cs;
+    }
+}
+
+/* the source alone: S(5,nDOF) at time t (zero without one) */
+int orc_source(orc_ctx *c, double t, double *S)
+{
+    long nDOF = (long)c->n * c->n * c->n * c->nElems;
+    if (!c->src_func) {
+        memset(S, 0, (size_t)(5 * nDOF) * sizeof(double));
+        return 0;
+    }
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < nDOF; i++) source_point(c, &c->Elem_xGP[3 * i], t, &S[5 * i]);
+    return 0;
+}
+
+int orc_rhs(orc_ctx *c, const double *U, double *Ut) { return orc_rhs_t(c, U, Ut, 0.0); }
+
+/* DGTimeDerivative_weakForm(t) (src/dg/dg.f90:255-425) */
+int orc_rhs_t(orc_ctx *c, const double *U, double *Ut, double t)
 {
     int N = c->N, n = c->n;
     long nDOF = (long)n * n * n * c->nElems, nfn = (long)n * n * c->nSides;
@@ -833,12 +906,15 @@ int orc_rhs(orc_ctx *c, const double *U, double *Ut)
     }
     /* 11.5 SurfIntCons (no Jacobian) */
     surfint_device(c, 5, c->Flux_master, c->Flux_slave, 0, 0, Ut);
-    /* 12. Ut = -Ut; 14. ApplyJacobian (applyjacobian.t90:196-224) */
+    /* 12. Ut = -Ut; 13. CalcSource: Ut += Ut_src/sJ (exactfunc.f90:1106-1108); 14. ApplyJacobian
+     * (applyjacobian.t90:196-224) */
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < nDOF; i++) {
-        double sj = c->sJ[i];
+        double sj = c->sJ[i], S[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (c->src_func) source_point(c, &c->Elem_xGP[3 * i], t, S);
         for (int v = 0; v < 5; v++) {
             double u = Ut[5 * i + v] * -1.0;
+            if (c->src_func) u = u + S[v] / sj;
             Ut[5 * i + v] = u * sj;
         }
     }
@@ -895,8 +971,7 @@ int orc_lserk_step(orc_ctx *c, double *U, double *Ut_tmp, double t, double dt)
     for (int s = 0; s < 5; s++) bdt[s] = b[s] * dt;
     for (int s = 0; s < 5; s++) {
         double tStage = (s == 0) ? t : t + cc[s] * dt;
-        (void)tStage;
-        orc_rhs(c, U, c->Ut_stage);
+        orc_rhs_t(c, U, c->Ut_stage, tStage);
         double mRKA = (s == 0) ? 0.0 : -1.0 * A[s];
 #pragma omp parallel for schedule(static)
         for (long i = 0; i < nTot; i++) {

@@ -51,6 +51,12 @@ int orc_set_mesh(orc_ctx *c, const int *ElemToSide, const int *SideToElem, const
                  const double *NormVec, const double *TangVec1, const double *TangVec2, const double *SurfElem);
 /* DGTimeDerivative_weakForm (src/dg/dg.f90:255-425) */
 int orc_rhs(orc_ctx *c, const double *U, double *Ut);
+/* the same at time t, which enters only through the manufactured source */
+int orc_rhs_t(orc_ctx *c, const double *U, double *Ut, double t);
+/* CalcSource of exact function 4, 41, 42 or 43 (other values: none) with Elem_xGP(3,0:N,0:N,0:N,nElems) */
+int orc_set_source(orc_ctx *c, int exactFunc, const double *advVel, const double *Elem_xGP);
+/* the physical-space source alone, S(5,0:N,0:N,0:N,nElems) at time t */
+int orc_source(orc_ctx *c, double t, double *S);
 /* access to the intermediate arrays of the last orc_rhs call (reference layouts); which:
  * 0 U_master 1 U_slave 2 UPrim 3 gradUx 4 gradUy 5 gradUz 6 gradUx_master 7 gradUx_slave
  * 8 Flux_master 9 gradUy_master 10 gradUz_master 11 gradUy_slave 12 gradUz_slave */

@@ -41,6 +41,9 @@ def lib():
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_set_mesh.argtypes = [C.c_void_p, ip, ip, ip] + [dp] * 8
         L.orc_rhs.argtypes = [C.c_void_p, dp, dp]
+        L.orc_rhs_t.argtypes = [C.c_void_p, dp, dp, C.c_double]
+        L.orc_set_source.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.orc_source.argtypes = [C.c_void_p, C.c_double, dp]
         L.orc_array.restype = C.POINTER(C.c_double)
         L.orc_array.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_long)]
         L.orc_lserk_step.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_double]
@@ -124,10 +127,19 @@ class Operator:
             lib().orc_destroy(self.h)
             self.h = None
 
-    def rhs(self, U):
+    def rhs(self, U, t=0.0):
         Ut = np.zeros_like(F(U), order="F")
-        lib().orc_rhs(self.h, F(U), Ut)
+        lib().orc_rhs_t(self.h, F(U), Ut, float(t))
         return Ut
+
+    def set_source(self, exactFunc, advVel, Elem_xGP):
+        self._xgp = F(Elem_xGP)  # the C side keeps its own copy
+        lib().orc_set_source(self.h, int(exactFunc), F(np.asarray(advVel, dtype=np.float64).reshape(3)), self._xgp)
+
+    def source(self, t):
+        S = np.zeros((5, self.n, self.n, self.n, self.mesh.nElems), order="F")
+        lib().orc_source(self.h, float(t), S)
+        return S
 
     def array(self, which):
         cnt = C.c_long(0)

@@ -7,6 +7,10 @@ own regression checks, GPL-3.0 reference repository), and print their SHA-256.
                                        non-periodic boundaries, BC type 2)
   cartbox_State_0000000.000001000.h5   regressioncheck/checks/timedisc/freestream_3D/ (a DG_Solution state file,
                                        N=3, written by the reference at t=1e-6 for that mesh)
+  CART_HEX_PERIODIC_0{02,04,08,16}_3D_mesh.h5
+                                       regressioncheck/checks/convtest/h_3D/ (the h- and p-convergence meshes:
+                                       periodic [-1,1]^3 in two HOPR zones of 1x2x2 .. 8x16x16 elements, the second
+                                       zone built from mirrored corners; p_3D's 2^3 mesh is the same file)
 
 Usage: python tests/golden/copy_hdf5_fixtures.py [/path/to/reference]
 """
@@ -17,7 +21,11 @@ import sys
 
 FILES = ("regressioncheck/checks/tgv/split/CART_HEX_PERIODIC_008_mesh.h5",
          "regressioncheck/checks/timedisc/freestream_3D/cartbox3D_mesh.h5",
-         "regressioncheck/checks/timedisc/freestream_3D/cartbox_State_0000000.000001000.h5")
+         "regressioncheck/checks/timedisc/freestream_3D/cartbox_State_0000000.000001000.h5",
+         "regressioncheck/checks/convtest/h_3D/CART_HEX_PERIODIC_002_3D_mesh.h5",
+         "regressioncheck/checks/convtest/h_3D/CART_HEX_PERIODIC_004_3D_mesh.h5",
+         "regressioncheck/checks/convtest/h_3D/CART_HEX_PERIODIC_008_3D_mesh.h5",
+         "regressioncheck/checks/convtest/h_3D/CART_HEX_PERIODIC_016_3D_mesh.h5")
 
 
 def main(ref_root):

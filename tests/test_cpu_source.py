@@ -1,0 +1,147 @@
+"""Manufactured source of the synthetic waves (ExactFunc 4, 41, 42, 43) and the error norms, on the CPU.
+
+What pins the oracle's source (CalcSource_CUDA_Kernel, src/equations/navierstokes/idealgas/exactfunc.f90:946-1111):
+the source is whatever makes the exact function (exactfunc.f90:326-420) solve the compressible Navier-Stokes
+equations, S = dU/dt + div(F_c(U) - F_v(U, grad U)).  The test evaluates that residual with 4th-order central
+differences in x and t of the exact function, pushed through the Euler and viscous fluxes written out here (constant
+mu, Stokes hypothesis, Fourier heat flux with lambda = mu cp / Pr), at random points and times, for each case and for
+two viscosities.  A dropped or mis-signed term, a wrong index (the wrong momentum component, the wrong AdvVel entry)
+or a wrong constant in the source leaves a residual of order 1e-3..1, far above the differences' error (below 1e-8).
+
+The operator with the source: at the exact solution U(t) the DG right-hand side converges to dU/dt as the mesh is
+refined (the truncation error of a consistent discretisation); without the source, or with it added at the wrong
+scale (reference instead of physical space) it would not.  Checked on 4^3 and 8^3 meshes at N=3.
+
+CalcErrorNorms (src/analyze/analyze.f90:383-479) against closed forms: a constant error c gives L2 = Linf = |c|, and
+the error x (exact 0) on [-1,1]^3 gives L2 = sqrt(1/3), Linf = 1.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from galaexi_amd import convtest as CT
+from galaexi_amd import mesh as M
+from galaexi_amd import testcases as T
+
+KAPPA, R, PR = 1.4, 287.058, 0.72
+
+
+def _d4(f, x, h):
+    """4th-order central difference of f at x (f maps x -> array)."""
+    return (-f(x + 2 * h) + 8 * f(x + h) - 8 * f(x - h) + f(x - 2 * h)) / (12 * h)
+
+
+def _prims(U):
+    rho = U[0]
+    vel = U[1:4] / rho
+    p = (KAPPA - 1.0) * (U[4] - 0.5 * rho * np.sum(vel * vel, axis=0))
+    return rho, vel, p, p / (rho * R)
+
+
+def ns_residual(case, adv, mu0, x, t, h=1e-3):
+    """dU/dt + sum_d d/dx_d (F_c,d - F_v,d) of the exact function at points x(3, M), times t(M)."""
+    U = lambda xx, tt: T.synthetic_wave(case, xx, tt, adv)
+    lam = mu0 * KAPPA * R / ((KAPPA - 1.0) * PR)
+    e = np.eye(3)[:, :, None]
+
+    def grads(xx, tt):
+        # d(u_j)/dx_i and dT/dx_i by differences of the exact function
+        du = np.empty((3, 3) + xx.shape[1:])
+        dT = np.empty((3,) + xx.shape[1:])
+        for i in range(3):
+            du[i] = _d4(lambda s: _prims(U(xx + s * e[i], tt))[1], 0.0, h)
+            dT[i] = _d4(lambda s: _prims(U(xx + s * e[i], tt))[3], 0.0, h)
+        return du, dT
+
+    def flux(d, xx, tt):
+        W = U(xx, tt)
+        rho, vel, p, _T = _prims(W)
+        du, dT = grads(xx, tt)
+        div = du[0, 0] + du[1, 1] + du[2, 2]
+        tau_d = np.array([mu0 * (du[d, j] + du[j, d]) - (2.0 / 3.0) * mu0 * div * (j == d) for j in range(3)])
+        Fc = np.array([W[1 + d], W[1] * vel[d] + p * (d == 0), W[2] * vel[d] + p * (d == 1),
+                       W[3] * vel[d] + p * (d == 2), (W[4] + p) * vel[d]])
+        Fv = np.array([np.zeros_like(rho), tau_d[0], tau_d[1], tau_d[2],
+                       np.sum(tau_d * vel, axis=0) + lam * dT[d]])
+        return Fc - Fv
+
+    res = _d4(lambda s: U(x, t + s), 0.0, h)
+    for d in range(3):
+        res = res + _d4(lambda s: flux(d, x + s * e[d], t), 0.0, h)
+    return res
+
+
+@pytest.mark.parametrize("case", T.SOURCE_FUNCS)
+@pytest.mark.parametrize("mu0", [1e-3, 0.05])
+def test_oracle_source_is_the_navier_stokes_residual(case, mu0):
+    rng = np.random.default_rng(case)
+    m = M.box_mesh((2, 2, 2), 1, lo=(-1.0, -1.0, -1.0), hi=(1.0, 1.0, 1.0))
+    adv = rng.uniform(0.1, 0.6, 3)
+    op = O.Operator(m, split=False, riemann=O.RIEMANN_ROEEF, kappa=KAPPA, R=R, Pr=PR, mu0=mu0)
+    xgp = rng.uniform(-1.0, 1.0, m.Elem_xGP.shape)  # any points: the source needs only the coordinates
+    op.set_source(case, adv, xgp)
+    for t in (0.0, 0.37, 1.9):
+        S = op.source(t).reshape(5, -1)
+        ref = ns_residual(case, adv, mu0, xgp.reshape(3, -1), np.full(S.shape[1], t))
+        err = np.abs(S - ref).max()
+        assert err < 1e-8 * max(1.0, np.abs(ref).max()), (case, mu0, t, err)
+        assert np.abs(S).max() > 1e-2
+
+
+def test_oracle_no_source_for_other_functions():
+    m = M.box_mesh((2, 2, 2), 1)
+    op = O.Operator(m, split=False, riemann=O.RIEMANN_ROEEF, kappa=KAPPA, R=R, Pr=PR, mu0=1e-3)
+    op.set_source(4, (0.3, 0.3, 0.3), m.Elem_xGP)
+    assert np.abs(op.source(0.2)).max() > 0
+    op.set_source(7, (0.3, 0.3, 0.3), m.Elem_xGP)  # CASE DEFAULT switches it off
+    assert np.abs(op.source(0.2)).max() == 0
+
+
+def test_oracle_rhs_with_source_converges_to_dudt():
+    case, adv, t, N = 4, (0.3, 0.3, 0.3), 0.41, 3
+    errs = []
+    for ne in (4, 8):
+        m = M.box_mesh((ne,) * 3, N, lo=(-1.0, -1.0, -1.0), hi=(1.0, 1.0, 1.0))
+        op = O.Operator(m, split=False, riemann=O.RIEMANN_ROEEF, kappa=KAPPA, R=R, Pr=PR, mu0=1e-3)
+        op.set_source(case, adv, m.Elem_xGP)
+        U = T.synthetic_wave(case, m.Elem_xGP, t, adv)
+        dudt = _d4(lambda s: T.synthetic_wave(case, m.Elem_xGP, t + s, adv), 0.0, 1e-3)
+        Ut = op.rhs(U, t)
+        errs.append(np.sqrt(np.mean((Ut - dudt) ** 2)))
+        if ne == 4:
+            Ut0 = O.Operator(m, split=False, riemann=O.RIEMANN_ROEEF, kappa=KAPPA, R=R, Pr=PR, mu0=1e-3).rhs(U, t)
+            assert np.sqrt(np.mean((Ut0 - dudt) ** 2)) > 50 * errs[0]  # without the source: no consistency
+    assert errs[0] / errs[1] > 2 ** N - 2, errs
+
+
+def test_error_norms_closed_forms():
+    for N in (2, 5):
+        m = M.box_mesh((2, 3, 2), N, lo=(-1.0, -1.0, -1.0), hi=(1.0, 1.0, 1.0))
+        en = CT.ErrorNorms(m)
+        assert en.NAnalyze == 2 * (N + 1)
+        assert abs(en.Vol - 8.0) < 1e-13
+        c = np.array([0.5, -2.0, 1e-3, 0.0, 3.0])
+        U = np.zeros((5,) + m.sJ.shape, order="F") + c[:, None, None, None, None]
+        L2, Linf = en(U, lambda x: np.zeros((5,) + x.shape[1:]))
+        assert np.allclose(L2, np.abs(c), rtol=1e-13, atol=1e-15) and np.allclose(Linf, np.abs(c), rtol=1e-13)
+        U = np.zeros((5,) + m.sJ.shape, order="F")
+        U[2] = m.Elem_xGP[0]
+        L2, Linf = en(U, lambda x: np.zeros((5,) + x.shape[1:]))
+        assert abs(L2[2] - np.sqrt(1.0 / 3.0)) < 1e-13 and abs(Linf[2] - 1.0) < 1e-13 and L2[0] == 0
+
+
+def test_error_norms_of_the_wave_itself_vanish_with_N():
+    m = M.box_mesh((2, 2, 2), 7, lo=(-1.0, -1.0, -1.0), hi=(1.0, 1.0, 1.0))
+    en = CT.ErrorNorms(m)
+    U = T.synthetic_wave(4, m.Elem_xGP, 0.3, (0.3, 0.3, 0.3))
+    L2, _ = en.wave(U, 4, 0.3, (0.3, 0.3, 0.3))
+    assert L2.max() < 3e-5  # interpolation error of the wave at N=7 (measured 8.8e-6 for E)
+    L2b, _ = en.wave(U, 4, 0.5, (0.3, 0.3, 0.3))  # the wrong time
+    assert L2b.min() > 1e-2
+
+
+def test_orders():
+    h = np.array([1.0, 0.5, 0.25])
+    e = 3.0 * h[:, None] ** np.array([4.0, 2.0])
+    assert np.allclose(CT.orders(h, e), [[4.0, 2.0], [4.0, 2.0]])
+    assert np.allclose(CT.orders(h, e[:, 0]), [4.0, 4.0])
