@@ -37,6 +37,20 @@
 #ifndef GALX_LDS_SWZ
 #define GALX_LDS_SWZ 0
 #endif
+// k_rhs: the node metrics issued with the state, ahead of the lifting (0 off, 1 mixed form only, 2 both forms)
+#ifndef GALX_PREFETCH_GEO
+#define GALX_PREFETCH_GEO 1
+#endif
+// k_lift: gradients of the surface nodes only (0 off, 1 FP64 form only, 2 both forms)
+#ifndef GALX_LIFT_SURF
+#define GALX_LIFT_SURF 1
+#endif
+// k_rhs, FP64 form: the LSERK register, the state and sJ stream into LDS by direct global->LDS loads during the
+// face tasks, so that the update does not wait on memory (1 on).  Measured on MI355X at N=7: within 0.3 % of the
+// plain path (the other workgroup on the CU already hides that latency), so off.
+#ifndef GALX_LDS_DMA
+#define GALX_LDS_DMA 0
+#endif
 
 namespace galx {
 
@@ -210,11 +224,22 @@ __device__ __forceinline__ void lift_face_fluxes(int e, int tid, const Mesh& mes
 
 // BR1 gradient g[12] = grad(u,v,w,T) (x, then y, then z) at volume node `node` of element e, from s_lift and the
 // lifting fluxes of lift_face_fluxes (the node half of lift_gradient; any thread may evaluate any node)
-template <int N, typename T, bool TR>
-__device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh, const Mats<N, T, TR>& M,
-                                              const T* s_lift, const T* s_lflux, T LHm, T LHp, T* g)
+template <int N, typename T>
+__device__ __forceinline__ void load_node_geo(int e, int node, const Mesh& mesh, T* gm)
 {
-    constexpr int n = N + 1, n2 = n * n, n3 = n2 * n;
+    constexpr int n3 = Dm<N>::n3;
+    const double* ge = mesh.geo + (long)e * 10 * n3 + node;
+#pragma unroll
+    for (int q = 0; q < 10; q++) gm[q] = T(ge[q * n3]);
+}
+
+// gm: the node's Ja^1(3), Ja^2(3), Ja^3(3), sJ (load_node_geo), loaded by the caller ahead of the barrier that
+// precedes this call so that their latency overlaps the face tasks
+template <int N, typename T, bool TR>
+__device__ __forceinline__ void node_gradient(int node, const Mats<N, T, TR>& M, const T* s_lift, const T* s_lflux,
+                                              T LHm, T LHp, const T* gm, T* g)
+{
+    constexpr int n = N + 1, n2 = n * n;
     constexpr int np = Lay<N>::np, n3p = Lay<N>::n3;
     const int i = node % n, j = (node / n) % n, k = node / n2;
     T gxi[4] = {0, 0, 0, 0}, get[4] = {0, 0, 0, 0}, gze[4] = {0, 0, 0, 0};
@@ -228,11 +253,8 @@ __device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh,
             gze[q] = gze[q] + dk * s_lift[q * n3p + Lay<N>::idx(i, j, l)];
         }
     }
-    const double* ge = mesh.geo + (long)e * 10 * n3 + node;
-    T mf[3] = {T(ge[0]), T(ge[n3]), T(ge[2 * n3])};
-    T mg[3] = {T(ge[3 * n3]), T(ge[4 * n3]), T(ge[5 * n3])};
-    T mh[3] = {T(ge[6 * n3]), T(ge[7 * n3]), T(ge[8 * n3])};
-    T sJ = T(ge[9 * n3]);
+    const T mf[3] = {gm[0], gm[1], gm[2]}, mg[3] = {gm[3], gm[4], gm[5]}, mh[3] = {gm[6], gm[7], gm[8]};
+    const T sJ = gm[9];
     T loc[12];
 #pragma unroll
     for (int q = 0; q < 12; q++) loc[q] = T(0);
@@ -262,14 +284,62 @@ __device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh,
     }
 }
 
+template <int N, typename T, bool TR>
+__device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh, const Mats<N, T, TR>& M,
+                                              const T* s_lift, const T* s_lflux, T LHm, T LHp, T* g)
+{
+    T gm[10];
+    load_node_geo<N, T>(e, node, mesh, gm);
+    node_gradient<N, T, TR>(node, M, s_lift, s_lflux, LHm, LHp, gm, g);
+}
+
+// nodes on the element surface, n^3 - (n-2)^3 of them: the k=0 plane, the rings of the inner k planes, the k=N plane
+template <int N>
+struct Surf {
+    static constexpr int n = N + 1, ring = 4 * N, count = 2 * n * n + (n - 2) * 4 * N;
+};
+
+template <int N>
+__host__ __device__ __forceinline__ int surface_node(int t)
+{
+    constexpr int n = N + 1, n2 = n * n, ring = Surf<N>::ring;
+    if (t < n2) return t;
+    t -= n2;
+    if (t < (n - 2) * ring) {
+        const int k = 1 + t / ring, q = t % ring;
+        int i, j;
+        if (q < n) {
+            i = q;
+            j = 0;
+        } else if (q < 2 * n) {
+            i = q - n;
+            j = N;
+        } else if (q < 3 * n - 2) {
+            i = 0;
+            j = 1 + q - 2 * n;
+        } else {
+            i = N;
+            j = 1 + q - (3 * n - 2);
+        }
+        return i + n * (j + n * k);
+    }
+    return t - (n - 2) * ring + n2 * N;
+}
+
 template <int N, typename T, typename TT, bool TR>
 __device__ __forceinline__ void lift_gradient(int e, int tid, const Mesh& mesh, const TT* __restrict__ utr,
                                               const Mats<N, T, TR>& M, const T* s_lift, T* s_lflux,
-                                              const Gas<T>& gas, T LHm, T LHp, T* g, T* s_nb = nullptr)
+                                              const Gas<T>& gas, T LHm, T LHp, T* g, T* s_nb = nullptr,
+                                              const T* gm = nullptr)
 {
     lift_face_fluxes<N, T, TT>(e, tid, mesh, utr, s_lift, s_lflux, gas, s_nb);
     __syncthreads();
-    if (tid < Dm<N>::n3) node_gradient<N, T, TR>(e, tid, mesh, M, s_lift, s_lflux, LHm, LHp, g);
+    if (tid < Dm<N>::n3) {
+        if (gm)
+            node_gradient<N, T, TR>(tid, M, s_lift, s_lflux, LHm, LHp, gm, g);
+        else
+            node_gradient<N, T, TR>(e, tid, mesh, M, s_lift, s_lflux, LHm, LHp, g);
+    }
 }
 
 // volume state of node tid: ConsToPrim (eos.f90:212-239) in FP64, then shifted by the pressure offset and
@@ -359,6 +429,14 @@ __global__ void __launch_bounds__(Dm<N>::block)
     load_mats<N, T>(M, mats, tid, D::block);
     const Gas<T> gas = make_gas<T>(gd);
     T uc[5], pr[6], H;
+    // only the surface nodes' gradients are used (for F_v.n on the faces): with `surf` the first Surf<N>::count
+    // threads evaluate them, one surface node each, and the interior nodes' gradients and metrics are skipped
+    // (measured on MI355X at N=7: 1.8 % faster in FP64, 3 % slower in the mixed form)
+    constexpr bool surf = GALX_LIFT_SURF == 2 || (GALX_LIFT_SURF == 1 && sizeof(T) == 8);
+    const bool gnode = surf ? tid < Surf<N>::count : tid < n3;
+    const int gn = surf ? (gnode ? surface_node<N>(tid) : 0) : tid;
+    T gm[10];
+    if (gnode) load_node_geo<N, T>(e, gn, mesh, gm);
     if (tid < n3) {
         load_state<N, T>(e, tid, U, gd, uc, pr, H);
         const int pt = Lay<N>::at(tid);
@@ -369,11 +447,13 @@ __global__ void __launch_bounds__(Dm<N>::block)
     }
     __syncthreads();
     T g[12];
-    lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_lift, s_lflux, gas, LHm, LHp, g);
+    lift_face_fluxes<N, T, TT>(e, tid, mesh, utr, s_lift, s_lflux, gas, nullptr);
+    __syncthreads();
+    if (gnode) node_gradient<N, T>(gn, M, s_lift, s_lflux, LHm, LHp, gm, g);
     __syncthreads();
     T* s_g = s_buf + 4 * n3p;  // [6][12][n2]
-    if (tid < n3) {
-        int i = tid % n, j = (tid / n) % n, k = tid / n2;
+    if (gnode) {
+        int i = gn % n, j = (gn / n) % n, k = gn / n2;
 #pragma unroll
         for (int ls = 1; ls <= 6; ls++) {
             int ab;
@@ -456,6 +536,22 @@ __global__ void __launch_bounds__(Dm<N>::block)
 // U is read and the new state written to Uout, which may be U itself (in-place LSERK update) or another
 // buffer (the adjoint's recompute keeps every stage input).
 // ---------------------------------------------------------------------------------------------------------
+// copy `bytes` (a multiple of 16) from global g to LDS s with direct global->LDS loads (global_load_lds_dwordx4):
+// each wave instruction moves 1 KB, lane L's 16 bytes landing at s + 16 L; the block's waves share the chunks.  The
+// data is in LDS once the issuing wave's vector-memory counter drains (s_waitcnt before the consuming barrier).
+template <int BLOCK>
+__device__ __forceinline__ void dma_to_lds(const void* g, void* s, int bytes, int tid)
+{
+    typedef __attribute__((address_space(1))) void gvoid;
+    typedef __attribute__((address_space(3))) void lvoid;
+    const int lane = tid & 63;
+    for (int c = tid >> 6; c * 1024 < bytes; c += BLOCK / 64) {
+        const int off = c * 1024 + lane * 16;
+        if (off < bytes)
+            __builtin_amdgcn_global_load_lds((gvoid*)((const char*)g + off), (lvoid*)((char*)s + c * 1024), 16, 0, 0);
+    }
+}
+
 template <int N, typename T, typename TT, bool SPLIT, int RIEMANN>
 __device__ __forceinline__ void face_flux(const T* tro, const T* fvo, const T* trn, const T* fvn,
                                           const double* __restrict__ fg, int n2, int flip, const Gas<T>& gas, T* F)
@@ -501,11 +597,16 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     constexpr int np = Lay<N>::np, n3p = Lay<N>::n3;  // padded node layout of the line-read arrays
     constexpr int w_a = 4 * n3p + 6 * 7 * n2;         // lifting
     constexpr int w_c = SPLIT ? 13 * n3p : 5 * n3p;   // volume fluxes / split-form partner data
-    constexpr int w_d = 6 * 5 * n2;                   // face results, new traces
+    constexpr int w_d = 6 * 5 * n2;                   // new traces
+    // FP64 form: K, U and sJ of the element ([5][n3], [5][n3], [n3]) copied into s_w during the face tasks
+    constexpr bool dma = GALX_LDS_DMA && sizeof(T) == 8 && (n3 % 2) == 0;
+    constexpr int w_e = dma ? 11 * n3 : 0;
     constexpr int w_ac = w_a > w_c ? w_a : w_c;
-    constexpr int lds_w = w_ac > w_d ? w_ac : w_d;
-    __shared__ T s_w[lds_w];
-    __shared__ T s_nb[6 * 5 * n2];  // neighbour trace records, from the lifting to the face tasks
+    constexpr int w_de = w_d > w_e ? w_d : w_e;
+    constexpr int lds_w = w_ac > w_de ? w_ac : w_de;
+    __shared__ __align__(16) T s_w[lds_w];
+    __shared__ T s_nb[6 * 5 * n2];  // neighbour trace records from the lifting; the face tasks overwrite each of
+                                    // their records with their face result
     __shared__ Mats<N, T> M;
     const int e = blockIdx.x, tid = threadIdx.x;
     const bool active = tid < n3;
@@ -514,7 +615,12 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     load_mats<N, T>(M, mats, tid, D::block);
     const Gas<T> gas = make_gas<T>(gd);
     T uc[5], pr[6], Hme = T(0);
+    // the node's metrics, issued with the state so that the lifting does not wait for them (measured on MI355X at
+    // N=7: 8 % faster in the mixed form, 0.8 % slower in FP64)
+    constexpr bool pgeo = GALX_PREFETCH_GEO == 2 || (GALX_PREFETCH_GEO == 1 && sizeof(T) == 4);
+    T gm[10];
     if (active) {
+        if (pgeo) load_node_geo<N, T>(e, tid, mesh, gm);
         load_state<N, T>(e, tid, U, gd, uc, pr, Hme);
         s_w[pt] = pr[1];
         s_w[n3p + pt] = pr[2];
@@ -525,7 +631,8 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     Stress<T> s;
     {
         T g[12];
-        lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3p, gas, LHm, LHp, g, s_nb);
+        lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3p, gas, LHm, LHp, g, s_nb,
+                                pgeo ? gm : nullptr);
         if (active) stress(pr, g, gas, s);
     }
     __syncthreads();
@@ -622,8 +729,17 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         for (int v = 0; v < 5; v++) ut[v] = ut[v] + tmp[v];
     }
 
+    // FP64 LSERK stage: the update's operands stream into LDS while the face tasks run, so that their memory
+    // latency overlaps the face fluxes instead of following the face barrier (no registers held)
+    const bool dma_now = dma && st.mode != 0;
+    if (dma_now) {
+        dma_to_lds<D::block>(Kreg + (long)e * 5 * n3, s_w, 5 * n3 * 8, tid);
+        dma_to_lds<D::block>(U + (long)e * 5 * n3, s_w + 5 * n3, 5 * n3 * 8, tid);
+        dma_to_lds<D::block>(mesh.geo + ((long)e * 10 + 9) * n3, s_w + 10 * n3, n3 * 8, tid);
+    }
+
     // ---- face fluxes (FillFlux) and surface integral (SurfInt_Device) ----
-    T* s_res = s_w;  // [6][5][n2]
+    T* s_res = dma ? s_nb : s_w;  // [6][5][n2]; in place of the task's own neighbour record when s_w is taken
     for (int t = tid; t < D::nf; t += D::block) {
         int ls = t / n2 + 1, ab = t - (ls - 1) * n2, a = ab % n, b = ab / n;
         int node = face_vol_node<N>(ls, a, b);
@@ -657,6 +773,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll
         for (int v = 0; v < 5; v++) s_res[(ls - 1) * 5 * n2 + v * n2 + ab] = sg * F[v];
     }
+    if (dma_now) __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed before the barrier publishes them
     __syncthreads();
     double unew[5];
     if (active) {
@@ -669,10 +786,29 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll
             for (int v = 0; v < 5; v++) loc[v] = loc[v] + s_res[(ls - 1) * 5 * n2 + v * n2 + ab] * LH;
         }
-        double sJd = ge[9 * n3];
+        // K, U and sJ from the LDS copies (FP64 LSERK stage) or from memory
         double* ut_out = (st.mode == 0 ? Ut : Kreg) + (long)e * 5 * n3 + tid;
-        const double* uo = U + (long)e * 5 * n3 + tid;
         double* un = Uout + (long)e * 5 * n3 + tid;
+        double kv[5] = {0, 0, 0, 0, 0}, uv[5] = {0, 0, 0, 0, 0}, sJd;
+        if (dma_now) {
+            const double* sw = reinterpret_cast<const double*>(s_w) + tid;
+            sJd = sw[10 * n3];
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                kv[v] = sw[v * n3];
+                uv[v] = sw[(5 + v) * n3];
+            }
+        } else {
+            sJd = ge[9 * n3];
+            if (st.mode != 0) {
+                const double* uo = U + (long)e * 5 * n3 + tid;
+#pragma unroll
+                for (int v = 0; v < 5; v++) {
+                    kv[v] = ut_out[v * n3];
+                    uv[v] = uo[v * n3];
+                }
+            }
+        }
 #pragma unroll
         for (int v = 0; v < 5; v++) {
             T tot = ut[v] + loc[v];
@@ -680,9 +816,9 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
             if (st.mode == 0) {
                 ut_out[v * n3] = utd;
             } else {
-                double kk = ut_out[v * n3] * st.mRKA + utd;
+                double kk = kv[v] * st.mRKA + utd;
                 ut_out[v * n3] = kk;
-                unew[v] = uo[v * n3] + kk * st.bdt;
+                unew[v] = uv[v] + kk * st.bdt;
                 un[v * n3] = unew[v];
             }
         }
