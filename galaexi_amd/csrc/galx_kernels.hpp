@@ -51,6 +51,11 @@
 #ifndef GALX_LDS_DMA
 #define GALX_LDS_DMA 0
 #endif
+// k_rhs, mixed split form: the nine metric terms of every node stay in LDS (FP32) from the start, so the volume
+// terms read them there instead of from memory a second time (1 on)
+#ifndef GALX_GEO_LDS
+#define GALX_GEO_LDS 1
+#endif
 
 namespace galx {
 
@@ -158,6 +163,22 @@ struct Mats {
     __device__ __forceinline__ T dvs(int a, int b) const { return DVS[off(a, b)]; }
 };
 
+// D_T alone, for a kernel that only differentiates (k_lift): 1 KB less LDS at N=7 in FP64
+template <int N, typename T, bool TR = (GALX_MATS_T != 0)>
+struct MatsD {
+    static constexpr int ld = Lay<N>::np;
+    T DT[ld * (N + 1)];
+    __device__ static __forceinline__ int off(int a, int b) { return TR ? b + ld * a : a + ld * b; }
+    __device__ __forceinline__ T dt(int a, int b) const { return DT[off(a, b)]; }
+};
+
+template <int N, typename T, bool TR>
+__device__ __forceinline__ void load_mats(MatsD<N, T, TR>& M, const T* __restrict__ g, int tid, int nthreads)
+{
+    constexpr int n = N + 1, nn = n * n;
+    for (int x = tid; x < nn; x += nthreads) M.DT[MatsD<N, T, TR>::off(x % n, x / n)] = g[x];
+}
+
 template <int N, typename T, bool TR>
 __device__ __forceinline__ void load_mats(Mats<N, T, TR>& M, const T* __restrict__ g, int tid, int nthreads)
 {
@@ -235,8 +256,8 @@ __device__ __forceinline__ void load_node_geo(int e, int node, const Mesh& mesh,
 
 // gm: the node's Ja^1(3), Ja^2(3), Ja^3(3), sJ (load_node_geo), loaded by the caller ahead of the barrier that
 // precedes this call so that their latency overlaps the face tasks
-template <int N, typename T, bool TR>
-__device__ __forceinline__ void node_gradient(int node, const Mats<N, T, TR>& M, const T* s_lift, const T* s_lflux,
+template <int N, typename T, class MT>
+__device__ __forceinline__ void node_gradient(int node, const MT& M, const T* s_lift, const T* s_lflux,
                                               T LHm, T LHp, const T* gm, T* g)
 {
     constexpr int n = N + 1, n2 = n * n;
@@ -290,7 +311,7 @@ __device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh,
 {
     T gm[10];
     load_node_geo<N, T>(e, node, mesh, gm);
-    node_gradient<N, T, TR>(node, M, s_lift, s_lflux, LHm, LHp, gm, g);
+    node_gradient<N, T>(node, M, s_lift, s_lflux, LHm, LHp, gm, g);
 }
 
 // nodes on the element surface, n^3 - (n-2)^3 of them: the k=0 plane, the rings of the inner k planes, the k=N plane
@@ -336,7 +357,7 @@ __device__ __forceinline__ void lift_gradient(int e, int tid, const Mesh& mesh, 
     __syncthreads();
     if (tid < Dm<N>::n3) {
         if (gm)
-            node_gradient<N, T, TR>(tid, M, s_lift, s_lflux, LHm, LHp, gm, g);
+            node_gradient<N, T>(tid, M, s_lift, s_lflux, LHm, LHp, gm, g);
         else
             node_gradient<N, T, TR>(e, tid, mesh, M, s_lift, s_lflux, LHm, LHp, g);
     }
@@ -413,8 +434,23 @@ __global__ void __launch_bounds__(Dm<N>::block)
 // thread forms its gradient (eight waves in parallel), and the face-node gradients reach the face tasks through
 // LDS: measured faster than face tasks that each evaluate their node's gradient (fewer, longer waves).
 // ---------------------------------------------------------------------------------------------------------
+// minimum waves per SIMD for k_lift: the mixed form's LDS (27 KB at N=7) lets four 512-thread workgroups share a
+// CU at 8 waves per SIMD (at most 64 registers, no spills): 2 % faster on MI355X at N=7.  FP64 stays at the
+// compiler's choice (120 registers): 6 waves fit the LDS but spill, and measured 37 % slower.  0 = compiler's choice
+#ifndef GALX_LIFT_MIXED_WAVES
+#define GALX_LIFT_MIXED_WAVES 8
+#endif
+#ifndef GALX_LIFT_FP64_WAVES
+#define GALX_LIFT_FP64_WAVES 0
+#endif
+template <int N, typename T>
+struct LiftWaves {
+    static constexpr int w = sizeof(T) == 4 ? GALX_LIFT_MIXED_WAVES : GALX_LIFT_FP64_WAVES;
+    static constexpr int value = (w > 0 && Dm<N>::block >= 256) ? w : 1;
+};
+
 template <int N, typename T, typename TT>
-__global__ void __launch_bounds__(Dm<N>::block)
+__global__ void __launch_bounds__(Dm<N>::block, (LiftWaves<N, T>::value))
     k_lift(const double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, TT* __restrict__ vtr,
            const T* __restrict__ mats, GasD gd, T LHm, T LHp)
 {
@@ -422,7 +458,7 @@ __global__ void __launch_bounds__(Dm<N>::block)
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
     constexpr int n3p = Lay<N>::n3;
     __shared__ T s_buf[4 * n3p + 6 * 12 * n2];  // own (u,v,w,T) | lifting fluxes, then own face gradients
-    __shared__ Mats<N, T> M;
+    __shared__ MatsD<N, T> M;
     const int e = blockIdx.x, tid = threadIdx.x;
     T* s_lift = s_buf;
     T* s_lflux = s_buf + 4 * n3p;
@@ -596,7 +632,10 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
     constexpr int np = Lay<N>::np, n3p = Lay<N>::n3;  // padded node layout of the line-read arrays
     constexpr int w_a = 4 * n3p + 6 * 7 * n2;         // lifting
-    constexpr int w_c = SPLIT ? 13 * n3p : 5 * n3p;   // volume fluxes / split-form partner data
+    // mixed split form: the metrics of all three directions in s_geo for the whole kernel (needs the prefetch)
+    constexpr bool geo_lds = GALX_GEO_LDS && SPLIT && sizeof(T) == 4 &&
+                             (GALX_PREFETCH_GEO == 2 || GALX_PREFETCH_GEO == 1);
+    constexpr int w_c = SPLIT ? (geo_lds ? 10 : 13) * n3p : 5 * n3p;  // volume fluxes / split-form partner data
     constexpr int w_d = 6 * 5 * n2;                   // new traces
     // FP64 form: K, U and sJ of the element ([5][n3], [5][n3], [n3]) copied into s_w during the face tasks
     constexpr bool dma = GALX_LDS_DMA && sizeof(T) == 8 && (n3 % 2) == 0;
@@ -605,6 +644,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     constexpr int w_de = w_d > w_e ? w_d : w_e;
     constexpr int lds_w = w_ac > w_de ? w_ac : w_de;
     __shared__ __align__(16) T s_w[lds_w];
+    __shared__ T s_geo[geo_lds ? 9 * n3p : 1];  // [3 directions][3][n3p] metrics Ja^1, Ja^2, Ja^3
     __shared__ T s_nb[6 * 5 * n2];  // neighbour trace records from the lifting; the face tasks overwrite each of
                                     // their records with their face result
     __shared__ Mats<N, T> M;
@@ -622,6 +662,10 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     if (active) {
         if (pgeo) load_node_geo<N, T>(e, tid, mesh, gm);
         load_state<N, T>(e, tid, U, gd, uc, pr, Hme);
+        if (geo_lds) {
+#pragma unroll
+            for (int c = 0; c < 9; c++) s_geo[c * n3p + pt] = gm[c];
+        }
         s_w[pt] = pr[1];
         s_w[n3p + pt] = pr[2];
         s_w[2 * n3p + pt] = pr[3];
@@ -669,9 +713,10 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         // ---- per direction: viscous contravariant flux with the weak-form D_Hat_T contraction
         // (VolInt_weakForm_Visc) and the split-form flux differencing (VolInt_splitForm_Kernel); the direction's
         // metric is read once for both ----
-        T* s_v = s_w;            // [6][n3p] rho,u,v,w,p,H
-        T* s_m = s_w + 6 * n3p;  // [3][n3p] metric of the current direction
-        T* s_f = s_w + 9 * n3p;  // [4][n3p] viscous contravariant flux (momentum, energy) of the current direction
+        T* s_v = s_w;                                // [6][n3p] rho,u,v,w,p,H
+        T* s_m = s_w + 6 * n3p;                      // [3][n3p] metric of the current direction
+        T* s_f = s_w + (geo_lds ? 6 : 9) * n3p;      // [4][n3p] viscous contravariant flux (momentum, energy) of
+                                                     // the current direction
         if (active) {
             s_v[pt] = pr[0];
             s_v[n3p + pt] = pr[1];
@@ -684,13 +729,20 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll 1
         for (int d = 0; d < 3; d++) {
             T mme[3] = {0, 0, 0};
+            if (geo_lds) s_m = s_geo + 3 * d * n3p;
             if (active) {
-                mme[0] = T(ge[(3 * d) * n3]);
-                mme[1] = T(ge[(3 * d + 1) * n3]);
-                mme[2] = T(ge[(3 * d + 2) * n3]);
-                s_m[pt] = mme[0];
-                s_m[n3p + pt] = mme[1];
-                s_m[2 * n3p + pt] = mme[2];
+                if (geo_lds) {
+                    mme[0] = s_m[pt];
+                    mme[1] = s_m[n3p + pt];
+                    mme[2] = s_m[2 * n3p + pt];
+                } else {
+                    mme[0] = T(ge[(3 * d) * n3]);
+                    mme[1] = T(ge[(3 * d + 1) * n3]);
+                    mme[2] = T(ge[(3 * d + 2) * n3]);
+                    s_m[pt] = mme[0];
+                    s_m[n3p + pt] = mme[1];
+                    s_m[2 * n3p + pt] = mme[2];
+                }
                 T f[5];
                 transformed_flux<T, false>(uc, pr, s, mme, gas.hshift, f);
 #pragma unroll
@@ -749,6 +801,8 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
         T Uo[5], fvo[4], Un[5], fvn[4];
         {
+            // from U at the face node (in L2 since the state load), not from the own slot of the trace buffer:
+            // the same record, but that read goes to memory (measured 1.5 % slower in FP64 on MI355X)
             const double* uo = U + (long)e * 5 * n3 + node;
             double c[5], tr[5];
 #pragma unroll
