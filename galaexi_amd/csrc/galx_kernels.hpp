@@ -664,7 +664,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         load_state<N, T>(e, tid, U, gd, uc, pr, Hme);
         if (geo_lds) {
 #pragma unroll
-            for (int c = 0; c < 9; c++) s_geo[c * n3p + pt] = gm[c];
+            for (int c = 0; c < 9; c++) s_geo[c * n3p + pt] = T(0.5) * gm[c];  // halves: see the split loop
         }
         s_w[pt] = pr[1];
         s_w[n3p + pt] = pr[2];
@@ -713,35 +713,37 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         // ---- per direction: viscous contravariant flux with the weak-form D_Hat_T contraction
         // (VolInt_weakForm_Visc) and the split-form flux differencing (VolInt_splitForm_Kernel); the direction's
         // metric is read once for both ----
-        T* s_v = s_w;                                // [6][n3p] rho,u,v,w,p,H
-        T* s_m = s_w + 6 * n3p;                      // [3][n3p] metric of the current direction
+        // the split flux runs on halves (split_volume_flux_pi_half): s_v and s_m hold half of each node's
+        // (rho,u,v,w,p,H) and metric, and the sum is doubled once at the end (exact)
+        T* s_v = s_w;                                // [6][n3p] rho,u,v,w,p,H (halves)
+        T* s_m = s_w + 6 * n3p;                      // [3][n3p] metric of the current direction (halves)
         T* s_f = s_w + (geo_lds ? 6 : 9) * n3p;      // [4][n3p] viscous contravariant flux (momentum, energy) of
                                                      // the current direction
+        const T hv[6] = {T(0.5) * pr[0], T(0.5) * pr[1], T(0.5) * pr[2], T(0.5) * pr[3], T(0.5) * pr[4],
+                         T(0.5) * Hme};
         if (active) {
-            s_v[pt] = pr[0];
-            s_v[n3p + pt] = pr[1];
-            s_v[2 * n3p + pt] = pr[2];
-            s_v[3 * n3p + pt] = pr[3];
-            s_v[4 * n3p + pt] = pr[4];
-            s_v[5 * n3p + pt] = Hme;
+#pragma unroll
+            for (int q = 0; q < 6; q++) s_v[q * n3p + pt] = hv[q];
         }
         T tmp[5] = {0, 0, 0, 0, 0};
 #pragma unroll 1
         for (int d = 0; d < 3; d++) {
-            T mme[3] = {0, 0, 0};
+            T mme[3] = {0, 0, 0}, hm[3] = {0, 0, 0};
             if (geo_lds) s_m = s_geo + 3 * d * n3p;
             if (active) {
                 if (geo_lds) {
-                    mme[0] = s_m[pt];
-                    mme[1] = s_m[n3p + pt];
-                    mme[2] = s_m[2 * n3p + pt];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        hm[c] = s_m[c * n3p + pt];
+                        mme[c] = T(2) * hm[c];
+                    }
                 } else {
-                    mme[0] = T(ge[(3 * d) * n3]);
-                    mme[1] = T(ge[(3 * d + 1) * n3]);
-                    mme[2] = T(ge[(3 * d + 2) * n3]);
-                    s_m[pt] = mme[0];
-                    s_m[n3p + pt] = mme[1];
-                    s_m[2 * n3p + pt] = mme[2];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        mme[c] = T(ge[(3 * d + c) * n3]);
+                        hm[c] = T(0.5) * mme[c];
+                        s_m[c * n3p + pt] = hm[c];
+                    }
                 }
                 T f[5];
                 transformed_flux<T, false>(uc, pr, s, mme, gas.hshift, f);
@@ -766,19 +768,19 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                 for (int l = 0; l <= N; l++) {
                     const int o = ln(l);
                     T Mo[3] = {s_m[o], s_m[n3p + o], s_m[2 * n3p + o]};
-                    T F[5];
-                    split_volume_flux_pi<T>(pr[0], pr[1], pr[2], pr[3], pr[4], Hme, s_v[o], s_v[n3p + o],
-                                            s_v[2 * n3p + o], s_v[3 * n3p + o], s_v[4 * n3p + o], s_v[5 * n3p + o],
-                                            mme, Mo, F);
+                    T G[5];
+                    split_volume_flux_pi_half<T>(hv[0], hv[1], hv[2], hv[3], hv[4], hv[5], s_v[o], s_v[n3p + o],
+                                                 s_v[2 * n3p + o], s_v[3 * n3p + o], s_v[4 * n3p + o],
+                                                 s_v[5 * n3p + o], hm, Mo, G);
                     T dv = M.dvs(l, idx);
 #pragma unroll
-                    for (int v = 0; v < 5; v++) tmp[v] = tmp[v] + dv * F[v];
+                    for (int v = 0; v < 5; v++) tmp[v] = tmp[v] + dv * G[v];
                 }
             }
             __syncthreads();
         }
 #pragma unroll
-        for (int v = 0; v < 5; v++) ut[v] = ut[v] + tmp[v];
+        for (int v = 0; v < 5; v++) ut[v] = ut[v] + T(2) * tmp[v];
     }
 
     // FP64 LSERK stage: the update's operands stream into LDS while the face tasks run, so that their memory
