@@ -145,3 +145,38 @@ def test_orders():
     e = 3.0 * h[:, None] ** np.array([4.0, 2.0])
     assert np.allclose(CT.orders(h, e), [[4.0, 2.0], [4.0, 2.0]])
     assert np.allclose(CT.orders(h, e[:, 0]), [4.0, 4.0])
+
+
+class _StepRecorder:
+    """An operator stand-in for convtest.run: CALCTIMESTEP returns a fixed dt; the steps taken are recorded."""
+
+    def __init__(self, dt_cfl, err_at=None):
+        self.dt_cfl, self.err_at, self.steps = dt_cfl, err_at, []
+
+    def set_state(self, U):
+        self.U = U
+
+    def get_state(self):
+        return self.U
+
+    def CalcTimeStep(self):
+        return self.dt_cfl, (2 if self.err_at is not None and len(self.steps) == self.err_at else 0)
+
+    def TimeStepByLSERKW2(self, t, dt, n):
+        self.steps.append((t, dt))
+
+
+def test_time_loop_reaches_tend_by_the_reference_rules():
+    """UpdateTimeStep (timedisc_func.f90:246-296): dt = min(dt_cfl, tend - t); a step that would leave less than 1 % of
+    dt before tend is stretched to reach it."""
+    op = _StepRecorder(0.3)
+    _, n = CT.run(op, np.zeros(1), 1.0)
+    assert n == 4 and [round(dt, 12) for _, dt in op.steps] == [0.3, 0.3, 0.3, 0.1]
+    assert abs(op.steps[-1][0] + op.steps[-1][1] - 1.0) < 1e-15
+    op = _StepRecorder(0.2495)  # 4 steps leave 0.002 < 1 % of dt: the fourth is stretched to end on tend
+    _, n = CT.run(op, np.zeros(1), 1.0)
+    assert n == 4 and abs(op.steps[-1][1] - (1.0 - 3 * 0.2495)) < 1e-15
+    op = _StepRecorder(0.25, err_at=2)
+    with pytest.raises(CT.TimeStepError) as ex:
+        CT.run(op, np.zeros(1), 1.0)
+    assert ex.value.step == 2 and ex.value.errType == 2 and abs(ex.value.t - 0.5) < 1e-15
