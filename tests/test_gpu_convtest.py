@@ -52,9 +52,10 @@ def conv_mesh(ne, N):
     return meshfile.read_mesh(os.path.join(GOLD, f"CART_HEX_PERIODIC_{ne:03d}_3D_mesh.h5"), N)
 
 
-def gpu_op(m, form, cfl, riemann="roeentropyfix", precision="fp64", case=4):
+def gpu_op(m, form, cfl, riemann="roeentropyfix", precision="fp64", case=4, mu0=GAS["mu0"]):
     from galaexi_amd import dg
-    op = dg.DGOperator(m.N, form=form, riemann=riemann, precision=precision, CFLScale=cfl, DFLScale=cfl, **GAS)
+    gas = dict(GAS, mu0=mu0)
+    op = dg.DGOperator(m.N, form=form, riemann=riemann, precision=precision, CFLScale=cfl, DFLScale=cfl, **gas)
     op.set_mesh(m)
     op.set_source(case, ADV, m.Elem_xGP)
     return op
@@ -63,8 +64,8 @@ def gpu_op(m, form, cfl, riemann="roeentropyfix", precision="fp64", case=4):
 class OracleRun:
     """The oracle behind the operator interface that convtest.run drives."""
 
-    def __init__(self, m, form, cfl, riemann="roeentropyfix", case=4):
-        self.op = O.Operator(m, split=form == "split", riemann=RIE[riemann], **GAS)
+    def __init__(self, m, form, cfl, riemann="roeentropyfix", case=4, mu0=GAS["mu0"]):
+        self.op = O.Operator(m, split=form == "split", riemann=RIE[riemann], **dict(GAS, mu0=mu0))
         self.op.set_source(case, ADV, m.Elem_xGP)
         self.cfl = cfl
 
@@ -193,14 +194,16 @@ def test_adjoint_with_source_vs_fd():
         o.close()
 
 
-@pytest.mark.parametrize("form", ["weak", "split"])
-def test_convtest_h_3d(form):
+@pytest.mark.parametrize("form,mu0", [("weak", 1e-3), ("split", 1e-3), ("weak", 0.0), ("split", 0.0)])
+def test_convtest_h_3d(form, mu0):
+    """mu0 = 0 is the Euler build (FLEXI_PARABOLIC=OFF in convtest/builds.ini): no viscous terms, no viscous time
+    step limit, and the source without its heat-flux term."""
     N, cells = 3, (2, 4, 8, 16)
-    say(f"case: convtest h_3D, {form} form, N={N}, meshes {cells}")
+    say(f"case: convtest h_3D, {form} form, mu0={mu0}, N={N}, meshes {cells}")
     errs = []
     for ne in cells:
         m = conv_mesh(ne, N)
-        op = gpu_op(m, form, 0.7)
+        op = gpu_op(m, form, 0.7, mu0=mu0)
         U0 = T.synthetic_wave(4, m.Elem_xGP, 0.0, ADV)
         U, steps = CT.run(op, U0, 1.0)
         op.close()
@@ -208,7 +211,7 @@ def test_convtest_h_3d(form):
         errs.append(L2)
         line = f"  {ne:2d}^3: {steps} steps, L2 {['%.3e' % e for e in L2]}"
         if ne <= 8:
-            Ur, steps_r = CT.run(OracleRun(m, form, 0.7), U0, 1.0)
+            Ur, steps_r = CT.run(OracleRun(m, form, 0.7, mu0=mu0), U0, 1.0)
             rel = rel_l2(U, Ur)
             line += f"; vs oracle run: rel L2 {rel:.1e}, {steps_r} steps"
             assert steps == steps_r and rel <= 1e-10
