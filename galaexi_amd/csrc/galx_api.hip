@@ -192,7 +192,8 @@ __global__ void __launch_bounds__(256) k_source(long nDOF, int n3, const double*
     }
 }
 
-// sum of the pressure over all nodes (element planes), for the pressure offset of the mixed form
+// per-block sums of the pressure over all nodes (element planes), for the pressure offset of the mixed form; the
+// host adds the block sums in order, so the offset, and with it the mixed-form trajectory, is the same every run
 __global__ void __launch_bounds__(256) k_pressure_sum(long nDOF, int n3, double kappa, const double* __restrict__ U,
                                                       double* __restrict__ out)
 {
@@ -210,7 +211,7 @@ __global__ void __launch_bounds__(256) k_pressure_sum(long nDOF, int n3, double 
     for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = p;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(out, s[0] + s[1] + s[2] + s[3]);
+    if (threadIdx.x == 0) out[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
 // ----------------------------------------------------------------------------------------------------------
@@ -414,6 +415,7 @@ struct galx_ctx {
     int srcFunc = 0;
     SourceCoefs src{};
     double* dSrcX = nullptr;
+    double* dPsum = nullptr;  // per-block pressure sums (mixed form's offset)
     long long bytes = 0;
     bool timing = false;
     std::map<std::string, Timer> timers;
@@ -571,7 +573,7 @@ int galx_destroy(galx_ctx* c)
                   c->dVtr, c->dMats, c->dDtc, c->dDtv, c->dDtOut, c->dErr};
     for (void* p : ps) dfree(p);
     void* pa[] = {c->dKbar,    c->dUbar,   c->dMutr,    c->dNghtr, c->dUfinal,
-                  c->dGrad,    c->dGradRef, c->dMats64, c->dTr64,  c->dSrcX};
+                  c->dGrad,    c->dGradRef, c->dMats64, c->dTr64,  c->dSrcX, c->dPsum};
     for (void* p : pa) dfree(p);
     for (double* p : c->ckpt) dfree(p);
     for (double* p : c->stages) dfree(p);
@@ -696,13 +698,16 @@ static int after_state(galx_ctx* c)
     c->pref = 0.0;
     if (c->cfg.precision == GALX_PREC_MIXED) {
         long nDOF = (long)c->nElems * c->n3;
-        HIPCHK(hipMemsetAsync(c->dDtOut, 0, sizeof(double), c->stream));
-        k_pressure_sum<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->cfg.kappa,
-                                                                                         c->dU, c->dDtOut);
+        const long nb = (nDOF + 255) / 256;
+        int rc;
+        if ((rc = dalloc(c, &c->dPsum, nb * sizeof(double)))) return rc;
+        k_pressure_sum<<<dim3((unsigned)nb), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->cfg.kappa, c->dU, c->dPsum);
         HIPCHK(hipGetLastError());
-        double sum = 0.0;
-        HIPCHK(hipMemcpyAsync(&sum, c->dDtOut, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        std::vector<double> part(nb);
+        HIPCHK(hipMemcpyAsync(part.data(), c->dPsum, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        double sum = 0.0;
+        for (double v : part) sum += v;
         c->pref = sum / (double)nDOF;
         if (!(c->pref == c->pref)) c->pref = 0.0;
     }
