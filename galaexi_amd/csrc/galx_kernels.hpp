@@ -45,12 +45,6 @@
 #ifndef GALX_LIFT_SURF
 #define GALX_LIFT_SURF 1
 #endif
-// k_rhs, FP64 form: the LSERK register, the state and sJ stream into LDS by direct global->LDS loads during the
-// face tasks, so that the update does not wait on memory (1 on).  Measured on MI355X at N=7: within 0.3 % of the
-// plain path (the other workgroup on the CU already hides that latency), so off.
-#ifndef GALX_LDS_DMA
-#define GALX_LDS_DMA 0
-#endif
 // k_rhs, mixed split form: the nine metric terms of every node stay in LDS (FP32) from the start, so the volume
 // terms read them there instead of from memory a second time (1 on)
 #ifndef GALX_GEO_LDS
@@ -572,22 +566,6 @@ __global__ void __launch_bounds__(Dm<N>::block)
 // U is read and the new state written to Uout, which may be U itself (in-place LSERK update) or another
 // buffer (the adjoint's recompute keeps every stage input).
 // ---------------------------------------------------------------------------------------------------------
-// copy `bytes` (a multiple of 16) from global g to LDS s with direct global->LDS loads (global_load_lds_dwordx4):
-// each wave instruction moves 1 KB, lane L's 16 bytes landing at s + 16 L; the block's waves share the chunks.  The
-// data is in LDS once the issuing wave's vector-memory counter drains (s_waitcnt before the consuming barrier).
-template <int BLOCK>
-__device__ __forceinline__ void dma_to_lds(const void* g, void* s, int bytes, int tid)
-{
-    typedef __attribute__((address_space(1))) void gvoid;
-    typedef __attribute__((address_space(3))) void lvoid;
-    const int lane = tid & 63;
-    for (int c = tid >> 6; c * 1024 < bytes; c += BLOCK / 64) {
-        const int off = c * 1024 + lane * 16;
-        if (off < bytes)
-            __builtin_amdgcn_global_load_lds((gvoid*)((const char*)g + off), (lvoid*)((char*)s + c * 1024), 16, 0, 0);
-    }
-}
-
 template <int N, typename T, typename TT, bool SPLIT, int RIEMANN>
 __device__ __forceinline__ void face_flux(const T* tro, const T* fvo, const T* trn, const T* fvn,
                                           const double* __restrict__ fg, int n2, int flip, const Gas<T>& gas, T* F)
@@ -636,17 +614,12 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     constexpr bool geo_lds = GALX_GEO_LDS && SPLIT && sizeof(T) == 4 &&
                              (GALX_PREFETCH_GEO == 2 || GALX_PREFETCH_GEO == 1);
     constexpr int w_c = SPLIT ? (geo_lds ? 10 : 13) * n3p : 5 * n3p;  // volume fluxes / split-form partner data
-    constexpr int w_d = 6 * 5 * n2;                   // new traces
-    // FP64 form: K, U and sJ of the element ([5][n3], [5][n3], [n3]) copied into s_w during the face tasks
-    constexpr bool dma = GALX_LDS_DMA && sizeof(T) == 8 && (n3 % 2) == 0;
-    constexpr int w_e = dma ? 11 * n3 : 0;
+    constexpr int w_d = 6 * 5 * n2;                   // face results, new traces
     constexpr int w_ac = w_a > w_c ? w_a : w_c;
-    constexpr int w_de = w_d > w_e ? w_d : w_e;
-    constexpr int lds_w = w_ac > w_de ? w_ac : w_de;
-    __shared__ __align__(16) T s_w[lds_w];
+    constexpr int lds_w = w_ac > w_d ? w_ac : w_d;
+    __shared__ T s_w[lds_w];
     __shared__ T s_geo[geo_lds ? 9 * n3p : 1];  // [3 directions][3][n3p] metrics Ja^1, Ja^2, Ja^3
-    __shared__ T s_nb[6 * 5 * n2];  // neighbour trace records from the lifting; the face tasks overwrite each of
-                                    // their records with their face result
+    __shared__ T s_nb[6 * 5 * n2];  // neighbour trace records, from the lifting to the face tasks
     __shared__ Mats<N, T> M;
     const int e = blockIdx.x, tid = threadIdx.x;
     const bool active = tid < n3;
@@ -783,17 +756,8 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         for (int v = 0; v < 5; v++) ut[v] = ut[v] + T(2) * tmp[v];
     }
 
-    // FP64 LSERK stage: the update's operands stream into LDS while the face tasks run, so that their memory
-    // latency overlaps the face fluxes instead of following the face barrier (no registers held)
-    const bool dma_now = dma && st.mode != 0;
-    if (dma_now) {
-        dma_to_lds<D::block>(Kreg + (long)e * 5 * n3, s_w, 5 * n3 * 8, tid);
-        dma_to_lds<D::block>(U + (long)e * 5 * n3, s_w + 5 * n3, 5 * n3 * 8, tid);
-        dma_to_lds<D::block>(mesh.geo + ((long)e * 10 + 9) * n3, s_w + 10 * n3, n3 * 8, tid);
-    }
-
     // ---- face fluxes (FillFlux) and surface integral (SurfInt_Device) ----
-    T* s_res = dma ? s_nb : s_w;  // [6][5][n2]; in place of the task's own neighbour record when s_w is taken
+    T* s_res = s_w;  // [6][5][n2]
     for (int t = tid; t < D::nf; t += D::block) {
         int ls = t / n2 + 1, ab = t - (ls - 1) * n2, a = ab % n, b = ab / n;
         int node = face_vol_node<N>(ls, a, b);
@@ -829,7 +793,6 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll
         for (int v = 0; v < 5; v++) s_res[(ls - 1) * 5 * n2 + v * n2 + ab] = sg * F[v];
     }
-    if (dma_now) __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed before the barrier publishes them
     __syncthreads();
     double unew[5];
     if (active) {
@@ -842,27 +805,16 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll
             for (int v = 0; v < 5; v++) loc[v] = loc[v] + s_res[(ls - 1) * 5 * n2 + v * n2 + ab] * LH;
         }
-        // K, U and sJ from the LDS copies (FP64 LSERK stage) or from memory
         double* ut_out = (st.mode == 0 ? Ut : Kreg) + (long)e * 5 * n3 + tid;
         double* un = Uout + (long)e * 5 * n3 + tid;
-        double kv[5] = {0, 0, 0, 0, 0}, uv[5] = {0, 0, 0, 0, 0}, sJd;
-        if (dma_now) {
-            const double* sw = reinterpret_cast<const double*>(s_w) + tid;
-            sJd = sw[10 * n3];
+        double kv[5] = {0, 0, 0, 0, 0}, uv[5] = {0, 0, 0, 0, 0};
+        const double sJd = ge[9 * n3];
+        if (st.mode != 0) {
+            const double* uo = U + (long)e * 5 * n3 + tid;
 #pragma unroll
             for (int v = 0; v < 5; v++) {
-                kv[v] = sw[v * n3];
-                uv[v] = sw[(5 + v) * n3];
-            }
-        } else {
-            sJd = ge[9 * n3];
-            if (st.mode != 0) {
-                const double* uo = U + (long)e * 5 * n3 + tid;
-#pragma unroll
-                for (int v = 0; v < 5; v++) {
-                    kv[v] = ut_out[v * n3];
-                    uv[v] = uo[v * n3];
-                }
+                kv[v] = ut_out[v * n3];
+                uv[v] = uo[v * n3];
             }
         }
 #pragma unroll
