@@ -113,7 +113,7 @@ __device__ __forceinline__ double max_weight(double mine, double other)
 __device__ __forceinline__ double sign0(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
 
 // adjoint of the Lax-Friedrichs flux with the PI central part (Riemann_LF riemann.f90:707-733 with
-// SplitSurfaceFluxPI splitflux.f90:735-761; the driver riemann.f90:216-289 rotates into the normal frame and back,
+// SplitSurfaceFluxPI splitflux.f90:735-761; the Riemann routine riemann.f90:216-289 rotates into the normal frame and back,
 // which leaves the flux unchanged, so the Cartesian form of the forward kernel is differentiated) with respect to
 // the state of one side (side 0: left/master, 1: right/slave).  Fb: cotangent of the Cartesian flux:
 //   F = (2q, q (u_L + u_R) + (p_L + p_R)/2 n, q (H_L + H_R)) - lmax/2 (U_R - U_L),

@@ -231,7 +231,7 @@ __device__ __forceinline__ void riemann_point(int solver, const T* UL, const T* 
 }
 
 // Riemann_LF with the SplitSurfaceFluxPI central part (riemann.f90:707-733, splitflux.f90:735-761) for the
-// Cartesian flux F.n, without the normal-frame rotation of the driver (riemann.f90:216-289): rotating into
+// Cartesian flux F.n, without the normal-frame rotation of the Riemann routine (riemann.f90:216-289): rotating into
 // (n,t1,t2) and back leaves the PI momentum flux 1/8 {rho} {u_n} {u} + 1/2 {p} n with {x} = x_L + x_R and the
 // dissipation -lmax/2 (U_R - U_L) unchanged, so only u.n is needed (algebraically identical to riemann_point).
 template <typename T>

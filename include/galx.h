@@ -148,8 +148,8 @@ int galx_lserk_adjoint(galx_ctx *ctx, double *Ubar);
 int galx_lserk_adjoint_device(galx_ctx *ctx, double *dUbar);
 /* galx_step_vjp: the adjoint of one TimeStepByLSERKW2 step (src/timedisc/timestep.f90:49-120) of size dt at the
  * current state U_n: Ubar holds dJ/dU_{n+1} on entry and dJ/dU_n = (dU_{n+1}/dU_n)^T dJ/dU_{n+1} on exit (U
- * layout, host / device).  The context stays at U_n; a trajectory recorded before is discarded.  The survey's
- * step VJP; a multi-step run is galx_lserk_forward_ckpt followed by galx_lserk_adjoint. */
+ * layout, host / device).  The context stays at U_n; a trajectory recorded before is discarded.  A multi-step
+ * run is galx_lserk_forward_ckpt followed by galx_lserk_adjoint. */
 int galx_step_vjp(galx_ctx *ctx, double t, double dt, double *Ubar);
 int galx_step_vjp_device(galx_ctx *ctx, double t, double dt, double *dUbar);
 

@@ -1,4 +1,4 @@
-"""Two survey configurations on the GPU at their own sizes.
+"""Two benchmark configurations on the GPU at their own sizes.
 
 C1: isentropic vortex (ExactFunc CASE(7), exactfunc.f90:482-503) with seeded centre in [-1,1]^2 and amplitude in
 [0.5,1], base flow rho=1, p=1/kappa, u=(0.5,0,0) (Ma 0.5), on the periodic box [-5,5]^3 with 8^3 elements, N=3

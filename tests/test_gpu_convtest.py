@@ -257,3 +257,27 @@ def test_weak_n2_diverges_like_the_oracle():
         CT.run(OracleRun(m, "weak", 0.25), U0, 1.0)
     say(f"  GPU: {eg.value}; oracle: {er.value}")
     assert eg.value.step == er.value.step and abs(eg.value.t - er.value.t) <= 1e-12
+
+
+def test_set_source_call_order():
+    """galx_set_source needs the mesh (GALX_E_STATE before it) and switches off for any other exact function."""
+    from galaexi_amd import dg
+    from galaexi_amd._lib import GalxError
+
+    say("case: set_source call order")
+    op = dg.DGOperator(3, form="weak", riemann="roeentropyfix", **GAS)
+    with pytest.raises(GalxError):
+        op.set_source(4, ADV)  # no mesh yet
+    m = conv_mesh(2, 3)
+    op.set_mesh(m)
+    op.set_source(0, ADV)  # off: no coordinates needed
+    U = T.synthetic_wave(4, m.Elem_xGP, 0.0, ADV)
+    op.set_state(U)
+    Ut0 = op.DGTimeDerivative_weakForm(0.2)
+    op.set_source(4, ADV, m.Elem_xGP)
+    Ut4 = op.DGTimeDerivative_weakForm(0.2)
+    ref = O.Operator(m, split=False, riemann=O.RIEMANN_ROEEF, **GAS)
+    ref.set_source(4, ADV, m.Elem_xGP)
+    S = ref.source(0.2)
+    assert rel_l2(Ut4 - Ut0, S) <= 1e-12  # the source is additive in physical space
+    op.close()

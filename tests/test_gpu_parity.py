@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the FP64 oracle and the reference's fixtures.
 
-Bar: bit-exact for the pure gathers (ProlongToFace); relative L2 <= 1e-12 for Ut in FP64 (north-star
-tolerance); the mixed form within the FP32 tolerance stated per test.
+Bar: bit-exact for the pure gathers (ProlongToFace); relative L2 <= 1e-12 for Ut in FP64 (the
+FP64 tolerance); the mixed form within the FP32 tolerance stated per test.
 """
 import os
 
