@@ -57,6 +57,32 @@ def build(degrees=DEFAULT_FP64, mixed_degrees=DEFAULT_MIXED, out=LIB, verbose=Fa
     return out
 
 
+FORTRAN_HOST_SRC = os.path.join(HERE, "..", "tests", "fortran", "galx_host.f90")
+FORTRAN_MODULE = os.path.join(HERE, "..", "include", "galx_mod.f90")
+FORTRAN_HOST = os.path.join(HERE, "..", "tests", "fortran", "galx_host")
+
+
+def build_fortran_host(out=FORTRAN_HOST, verbose=False):
+    """Compile the Fortran host test program against include/galx_mod.f90 and libgalx.so (rpath relative to the
+    binary, so the tree can move).  Returns the binary's path, or None when no Fortran compiler is installed."""
+    fc = os.environ.get("GALX_FC", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(fc):
+        print(f"no Fortran compiler at {fc}: the Fortran host test program is not built", flush=True)
+        return None
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as mod_dir:
+        cmd = [fc, "-O2", "-module-dir", mod_dir, "-o", out, FORTRAN_MODULE, FORTRAN_HOST_SRC,
+               "-L" + HERE, "-lgalx", "-Wl,-rpath,$ORIGIN/../../galaexi_amd"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout[-5000:] + res.stderr[-5000:])
+        raise RuntimeError("Fortran host build failed")
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--degrees", default=",".join(map(str, DEFAULT_FP64)))

@@ -105,6 +105,13 @@ INTERFACE
     REAL(C_DOUBLE),VALUE          :: t, dt
     REAL(C_DOUBLE),INTENT(INOUT)  :: Ubar(*)
   END FUNCTION
+  TYPE(C_PTR) FUNCTION galx_version() BIND(C,NAME='galx_version')
+    IMPORT :: C_PTR
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION galx_supported_degrees(precision) BIND(C,NAME='galx_supported_degrees')
+    IMPORT :: C_INT
+    INTEGER(C_INT),VALUE :: precision
+  END FUNCTION
   TYPE(C_PTR) FUNCTION galx_last_error() BIND(C,NAME='galx_last_error')
     IMPORT :: C_PTR
   END FUNCTION

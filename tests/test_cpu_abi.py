@@ -105,3 +105,22 @@ END PROGRAM use_galx
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(tmp_path / "use_galx")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and r.stdout.split()[0] == "-1", (r.stdout, r.stderr)
+
+
+def test_fortran_host_binds_the_library():
+    """The Fortran host test program (built against include/galx_mod.f90) loads libgalx.so and reads the version and
+    the degree masks through the ISO_C_BINDING interfaces; no device is touched."""
+    import os
+    import subprocess
+
+    host = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortran", "galx_host")
+    if not os.path.exists(host):
+        pytest.skip("Fortran host not built (no Fortran compiler at build time)")
+    from galaexi_amd import _lib
+
+    res = subprocess.run([host, "--version"], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0, res.stdout + res.stderr
+    L = _lib.lib()
+    assert L.galx_version().decode() in res.stdout
+    assert f"fp64 mask {L.galx_supported_degrees(0)}" in res.stdout
+    assert f"mixed mask {L.galx_supported_degrees(1)}" in res.stdout
