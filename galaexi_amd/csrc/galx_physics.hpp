@@ -22,8 +22,15 @@ struct Gas {
     T kappa, R, mu, lambda, pref, hshift;
 };
 
+// square root, reciprocal and quotient: IEEE-exact in FP64; in FP32 (the mixed form) the hardware instructions
+// (v_sqrt_f32, v_rcp_f32, about 1 ulp), whose error is far below the rounding of the FP32 fluxes they feed, at one
+// instruction instead of a correction sequence (the mixed kernels are VALU-bound)
 __device__ __forceinline__ double gsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float gsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ float gsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double grcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float grcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double gdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float gdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double gabs(double x) { return fabs(x); }
 __device__ __forceinline__ float gabs(float x) { return fabsf(x); }
 __device__ __forceinline__ double gmax(double a, double b) { return fmax(a, b); }
@@ -33,13 +40,13 @@ __device__ __forceinline__ float gmax(float a, float b) { return fmaxf(a, b); }
 template <typename T>
 __device__ __forceinline__ void cons_to_prim(const T* c, T* p, const Gas<T>& g)
 {
-    T sRho = T(1) / c[0];
+    T sRho = grcp(c[0]);
     p[0] = c[0];
     p[1] = c[1] * sRho;
     p[2] = c[2] * sRho;
     p[3] = c[3] * sRho;
     p[4] = (g.kappa - T(1)) * (c[4] - T(0.5) * (c[1] * p[1] + c[2] * p[2] + c[3] * p[3]));
-    p[5] = p[4] * sRho / g.R;
+    p[5] = gdiv(p[4] * sRho, g.R);
 }
 
 // face state from a trace record (rho, rho*u, rho*v, rho*w, p'): Uc = (rho, rho*u, rho*v, rho*w, E'),
@@ -47,18 +54,18 @@ __device__ __forceinline__ void cons_to_prim(const T* c, T* p, const Gas<T>& g)
 template <typename T>
 __device__ __forceinline__ void trace_state(const T* tr, const Gas<T>& g, T* Uc, T* P)
 {
-    T sRho = T(1) / tr[0];
+    T sRho = grcp(tr[0]);
     P[0] = tr[0];
     P[1] = tr[1] * sRho;
     P[2] = tr[2] * sRho;
     P[3] = tr[3] * sRho;
     P[4] = tr[4];
-    P[5] = (tr[4] + g.pref) * sRho / g.R;
+    P[5] = gdiv((tr[4] + g.pref) * sRho, g.R);
     Uc[0] = tr[0];
     Uc[1] = tr[1];
     Uc[2] = tr[2];
     Uc[3] = tr[3];
-    Uc[4] = tr[4] / (g.kappa - T(1)) + T(0.5) * (tr[1] * P[1] + tr[2] * P[2] + tr[3] * P[3]);
+    Uc[4] = gdiv(tr[4], g.kappa - T(1)) + T(0.5) * (tr[1] * P[1] + tr[2] * P[2] + tr[3] * P[3]);
 }
 
 // trace record of a conservative FP64 state (ConsToPrim pressure, shifted by pref)
@@ -84,7 +91,7 @@ template <typename T>
 __device__ __forceinline__ void rotate_state(const T* U, const T* P, const T* nv, const T* t1, const T* t2, Ext<T>& X)
 {
     X.dens = U[0];
-    X.srho = T(1) / X.dens;
+    X.srho = grcp(X.dens);
     X.ener = U[4];
     X.pres = P[4];
     X.vel[0] = P[1] * nv[0] + P[2] * nv[1] + P[3] * nv[2];
@@ -110,8 +117,8 @@ __device__ __forceinline__ void euler_flux_1d(const Ext<T>& X, T hshift, T* F)
 template <typename T>
 __device__ __forceinline__ void split_surface_flux_pi(const Ext<T>& L, const Ext<T>& R, T hshift, T* F)
 {
-    T HL = (L.ener + L.pres + hshift) / L.dens;
-    T HR = (R.ener + R.pres + hshift) / R.dens;
+    T HL = gdiv(L.ener + L.pres + hshift, L.dens);
+    T HR = gdiv(R.ener + R.pres + hshift, R.dens);
     T sr = L.dens + R.dens, su = L.vel[0] + R.vel[0];
     F[0] = T(0.25) * sr * su;
     F[1] = T(0.125) * sr * (su * su) + T(0.5) * (L.pres + R.pres);
@@ -151,7 +158,7 @@ __device__ __forceinline__ void riemann_solver(int solver, const Ext<T>& L, cons
     }
     T HL = (L.ener + L.pres + hshift) * L.srho, HR = (R.ener + R.pres + hshift) * R.srho;
     T sqL = gsqrt(L.dens), sqR = gsqrt(R.dens);
-    T ss = T(1) / (sqL + sqR);
+    T ss = grcp(sqL + sqR);
     T rv0 = (sqR * R.vel[0] + sqL * L.vel[0]) * ss;
     T rv1 = (sqR * R.vel[1] + sqL * L.vel[1]) * ss;
     T rv2 = (sqR * R.vel[2] + sqL * L.vel[2]) * ss;
@@ -167,8 +174,8 @@ __device__ __forceinline__ void riemann_solver(int solver, const Ext<T>& L, cons
         dU[5] = dU[4] - (dU[2] - rv1 * dU[0]) * rv1 - (dU[3] - rv2 * dU[0]) * rv2;
         al[2] = dU[2] - rv1 * dU[0];
         al[3] = dU[3] - rv2 * dU[0];
-        al[1] = (kappa - T(1)) / (Roec * Roec) * (dU[0] * (RoeH - rv0 * rv0) - dU[5] + rv0 * dU[1]);
-        al[0] = T(0.5) / Roec * (dU[0] * (rv0 + Roec) - dU[1] - Roec * al[1]);
+        al[1] = gdiv(kappa - T(1), Roec * Roec) * (dU[0] * (RoeH - rv0 * rv0) - dU[5] + rv0 * dU[1]);
+        al[0] = gdiv(T(0.5), Roec) * (dU[0] * (rv0 + Roec) - dU[1] - Roec * al[1]);
         al[4] = dU[0] - al[0] - al[1];
 #pragma unroll
         for (int v = 0; v < 5; v++) aa[v] = gabs(a[v]);
@@ -177,7 +184,7 @@ __device__ __forceinline__ void riemann_solver(int solver, const Ext<T>& L, cons
         T RoeDens = gsqrt(L.dens * R.dens);
         T d0 = R.dens - L.dens, d1 = R.vel[0] - L.vel[0], d2 = R.vel[1] - L.vel[1], d3 = R.vel[2] - L.vel[2],
           d4 = R.pres - L.pres;
-        T tmp = T(0.5) / (Roec * Roec);
+        T tmp = gdiv(T(0.5), Roec * Roec);
         al[0] = tmp * (d4 - RoeDens * Roec * d1);
         al[1] = d0 - d4 * T(2) * tmp;
         al[2] = RoeDens * d2;
@@ -188,7 +195,7 @@ __device__ __forceinline__ void riemann_solver(int solver, const Ext<T>& L, cons
 #pragma unroll
         for (int v = 0; v < 5; v++) {
             T da = gmax(T(0), gmax(a[v] - aL[v], aR[v] - a[v]));
-            aa[v] = (gabs(a[v]) < da) ? T(0.5) * (a[v] * a[v] / da + da) : gabs(a[v]);
+            aa[v] = (gabs(a[v]) < da) ? T(0.5) * (gdiv(a[v] * a[v], da) + da) : gabs(a[v]);
         }
     }
     T r1[5] = {T(1), a[0], rv1, rv2, RoeH - rv0 * Roec};
@@ -240,7 +247,7 @@ __device__ __forceinline__ void lf_pi_cartesian(const T* UL, const T* UR, const 
 {
     const T vnL = PL[1] * nv[0] + PL[2] * nv[1] + PL[3] * nv[2];
     const T vnR = PR[1] * nv[0] + PR[2] * nv[1] + PR[3] * nv[2];
-    const T sRhoL = T(1) / UL[0], sRhoR = T(1) / UR[0];
+    const T sRhoL = grcp(UL[0]), sRhoR = grcp(UR[0]);
     const T HL = (UL[4] + PL[4] + g.hshift) * sRhoL, HR = (UR[4] + PR[4] + g.hshift) * sRhoR;
     const T cL = gsqrt(g.kappa * (PL[4] + g.pref) * sRhoL), cR = gsqrt(g.kappa * (PR[4] + g.pref) * sRhoR);
     const T lmax = gmax(gabs(vnR), gabs(vnL)) + gmax(cL, cR);
@@ -311,7 +318,7 @@ template <typename T, bool EULER>
 __device__ __forceinline__ void transformed_flux(const T* U, const T* P, const Stress<T>& s, const T* m, T hshift, T* r)
 {
     if (EULER) {
-        T Ep = (U[4] + P[4] + hshift) / U[0];
+        T Ep = gdiv(U[4] + P[4] + hshift, U[0]);
         T Mmom = m[0] * U[1] + m[1] * U[2] + m[2] * U[3];
         r[0] = Mmom;
         r[1] = Mmom * P[1] + m[0] * P[4] - m[0] * s.txx - m[1] * s.txy - m[2] * s.txz;
