@@ -368,7 +368,9 @@ __device__ __forceinline__ void load_state(int e, int tid, const double* __restr
     double c[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) c[v] = u[v * n3];
-    double sRho = 1.0 / c[0];
+    // the mixed form rounds these to FP32: its 1/rho comes from the refined hardware reciprocal
+    constexpr bool fast = sizeof(T) == 4;
+    double sRho = fast ? rcp64_fast(c[0]) : 1.0 / c[0];
     double vx = c[1] * sRho, vy = c[2] * sRho, vz = c[3] * sRho;
     double p = (gd.kappa - 1.0) * (c[4] - 0.5 * (c[1] * vx + c[2] * vy + c[3] * vz));
     uc[0] = T(c[0]);
@@ -382,14 +384,14 @@ __device__ __forceinline__ void load_state(int e, int tid, const double* __restr
     pr[3] = T(vz);
     pr[4] = T(p - gd.pref);
     pr[5] = T(p * sRho / gd.R);
-    H = T((c[4] + p) / c[0]);
+    H = fast ? T((c[4] + p) * sRho) : T((c[4] + p) / c[0]);
 }
 
 template <typename TT>
 __device__ __forceinline__ void store_trace(TT* o, int n2, const double* c, const GasD& gd)
 {
     double tr[5];
-    trace_record(c, gd.kappa, gd.pref, tr);
+    trace_record<sizeof(TT) == 4>(c, gd.kappa, gd.pref, tr);
 #pragma unroll
     for (int v = 0; v < 5; v++) o[v * n2] = TT(tr[v]);
 }
@@ -773,7 +775,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
             double c[5], tr[5];
 #pragma unroll
             for (int v = 0; v < 5; v++) c[v] = uo[v * n3];
-            trace_record(c, gd.kappa, gd.pref, tr);
+            trace_record<sizeof(TT) == 4>(c, gd.kappa, gd.pref, tr);
 #pragma unroll
             for (int v = 0; v < 5; v++) Uo[v] = T(TT(tr[v]));  // exactly the record the neighbour reads
         }
@@ -837,7 +839,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     T* s_u = s_w;  // [6][5][n2] trace records of the new U, rounded to the trace type
     if (active) {
         double tr[5];
-        trace_record(unew, gd.kappa, gd.pref, tr);
+        trace_record<sizeof(TT) == 4>(unew, gd.kappa, gd.pref, tr);
 #pragma unroll
         for (int ls = 1; ls <= 6; ls++) {
             int ab;

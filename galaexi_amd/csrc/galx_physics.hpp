@@ -68,10 +68,20 @@ __device__ __forceinline__ void trace_state(const T* tr, const Gas<T>& g, T* Uc,
     Uc[4] = gdiv(tr[4], g.kappa - T(1)) + T(0.5) * (tr[1] * P[1] + tr[2] * P[2] + tr[3] * P[3]);
 }
 
-// trace record of a conservative FP64 state (ConsToPrim pressure, shifted by pref)
+// 1/x in FP64 from the hardware reciprocal (v_rcp_f64) refined by one Newton step: for FP64 values that are rounded
+// to FP32 afterwards (the mixed form's trace records and node states), three instructions instead of a division
+__device__ __forceinline__ double rcp64_fast(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// trace record of a conservative FP64 state (ConsToPrim pressure, shifted by pref); FAST for records stored in FP32
+template <bool FAST = false>
 __device__ __forceinline__ void trace_record(const double* c, double kappa, double pref, double* tr)
 {
-    double sRho = 1.0 / c[0];
+    double sRho = FAST ? rcp64_fast(c[0]) : 1.0 / c[0];
     double u = c[1] * sRho, v = c[2] * sRho, w = c[3] * sRho;
     double p = (kappa - 1.0) * (c[4] - 0.5 * (c[1] * u + c[2] * v + c[3] * w));
     tr[0] = c[0];
