@@ -31,16 +31,21 @@ NSTAGES = 5
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rhs_k_lift_tgv_n7_32cubed_fp64_traffic.json")
 
 
-def alg_bytes(N, b_t):
+def alg_bytes(N, b_t, face_geo=4):
     """Algorithmic HBM bytes per DOF for one stage's two kernels (b_t: bytes per trace value, 8 in FP64, 4 in the
-    mixed form; phi = 6 n^2 / n^3 face nodes per volume node):
-      k_lift: read U (5x8), metrics+sJ (10x8), face normal+SurfElem (phi 4x8), neighbour U traces (phi 5 b_t);
-              write own viscous normal-flux traces (phi 4 b_t)
-      k_rhs : read U (5x8), metrics+sJ (10x8), face geometry (phi 10x8), neighbour U and viscous-flux traces
+    mixed form; phi = 6 n^2 / n^3 face nodes per volume node; face_geo: face geometry values k_rhs reads per face
+    node, 4 (normal, SurfElem) for the split form with Lax-Friedrichs, which is formed in Cartesian components,
+    10 (normal, both tangents, SurfElem) for the solvers that rotate into the normal frame):
+      k_lift: read U (5x8), metrics+sJ at the surface nodes (sigma 10x8, sigma = (n^3 - (n-2)^3) / n^3), face
+              normal+SurfElem (phi 4x8), neighbour U traces (phi 5 b_t); write own viscous normal-flux traces
+              (phi 4 b_t)
+      k_rhs : read U (5x8), metrics+sJ (10x8), face geometry (phi face_geo x8), neighbour U and viscous-flux traces
               (phi 9 b_t), LSERK register (5x8); write U (5x8), register (5x8), own next U traces (phi 5 b_t)"""
-    phi = 6.0 / (N + 1)
-    lift = 5 * 8 + 10 * 8 + phi * (4 * 8 + 5 * b_t + 4 * b_t)
-    rhs = 5 * 8 + 10 * 8 + phi * 10 * 8 + phi * 9 * b_t + 5 * 8 + 5 * 8 + 5 * 8 + phi * 5 * b_t
+    n = N + 1
+    phi = 6.0 / n
+    sigma = (n ** 3 - max(n - 2, 0) ** 3) / n ** 3
+    lift = 5 * 8 + sigma * 10 * 8 + phi * (4 * 8 + 5 * b_t + 4 * b_t)
+    rhs = 5 * 8 + 10 * 8 + phi * face_geo * 8 + phi * 9 * b_t + 5 * 8 + 5 * 8 + 5 * 8 + phi * 5 * b_t
     return lift, rhs
 
 

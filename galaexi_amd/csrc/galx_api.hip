@@ -389,7 +389,7 @@ struct galx_ctx {
     double pref = 0.0;  // pressure offset of the mixed form (mean initial pressure); 0 in FP64
     // device buffers
     double *dU = nullptr, *dK = nullptr, *dUt = nullptr, *dScratch = nullptr;
-    double *dGeo = nullptr, *dFgeo = nullptr;
+    double *dGeo = nullptr, *dFgeo = nullptr, *dSgeo = nullptr;
     int *dConn = nullptr, *dInv = nullptr;
     void *dUtr[2] = {nullptr, nullptr}, *dVtr = nullptr, *dMats = nullptr;
     double *dDtc = nullptr, *dDtv = nullptr, *dDtOut = nullptr;
@@ -474,7 +474,7 @@ static void flush_timers(galx_ctx* c)
     }
 }
 
-static Mesh mesh_of(galx_ctx* c) { return Mesh{c->dGeo, c->dFgeo, c->dConn, c->dInv}; }
+static Mesh mesh_of(galx_ctx* c) { return Mesh{c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dSgeo}; }
 static GasD gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, c->pref}; }
 // the adjoint kernels and their FP64 trace records use the pressure itself (no offset)
 static GasD adj_gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, 0.0}; }
@@ -569,8 +569,8 @@ int galx_destroy(galx_ctx* c)
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     flush_timers(c);
-    void* ps[] = {c->dU, c->dK, c->dUt, c->dScratch, c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dUtr[0], c->dUtr[1],
-                  c->dVtr, c->dMats, c->dDtc, c->dDtv, c->dDtOut, c->dErr};
+    void* ps[] = {c->dU,      c->dK,    c->dUt,   c->dScratch, c->dGeo,  c->dFgeo, c->dSgeo, c->dConn, c->dInv,
+                  c->dUtr[0], c->dUtr[1], c->dVtr, c->dMats,    c->dDtc, c->dDtv,  c->dDtOut, c->dErr};
     for (void* p : ps) dfree(p);
     void* pa[] = {c->dKbar,    c->dUbar,   c->dMutr,    c->dNghtr, c->dUfinal,
                   c->dGrad,    c->dGradRef, c->dMats64, c->dTr64,  c->dSrcX, c->dPsum};
@@ -636,6 +636,8 @@ int galx_set_mesh(galx_ctx* c, int nElems, int nSides, const int* E2S, const int
     if ((rc = dalloc(c, &c->dK, 5 * nDOF * sizeof(double)))) return rc;
     if ((rc = dalloc(c, &c->dGeo, 10 * nDOF * sizeof(double)))) return rc;
     if ((rc = dalloc(c, &c->dFgeo, 10 * nfn * sizeof(double)))) return rc;
+    const int ns = surface_count(c->cfg.N);
+    if ((rc = dalloc(c, &c->dSgeo, 10L * nElems * ns * sizeof(double)))) return rc;
     if ((rc = dalloc(c, &c->dConn, 6L * nElems * sizeof(int)))) return rc;
     if ((rc = dalloc(c, &c->dInv, inv.size() * sizeof(int)))) return rc;
     if ((rc = dalloc(c, (char**)&c->dUtr[0], 2 * 5 * nfn * c->tsz))) return rc;
@@ -647,9 +649,12 @@ int galx_set_mesh(galx_ctx* c, int nElems, int nSides, const int* E2S, const int
     if ((rc = dalloc(c, &c->dErr, sizeof(int)))) return rc;
     HIPCHK(hipMemcpy(c->dConn, conn.data(), conn.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->dInv, inv.data(), inv.size() * sizeof(int), hipMemcpyHostToDevice));
-    // geometry into element / side planes, in chunks through pinned staging
+    // geometry into element / side planes, in chunks through pinned staging; the surface-node copy of the element
+    // geometry (k_lift's metrics) from the same planes
     const long chunkE = 4096;
-    std::vector<double> buf;
+    std::vector<double> buf, sbuf(10 * chunkE * ns);
+    std::vector<int> snode(ns);
+    for (int t = 0; t < ns; t++) snode[t] = surface_node_of(c->cfg.N, t);
     buf.resize(10 * chunkE * n3);
     for (long e0 = 0; e0 < nElems; e0 += chunkE) {
         long ne = std::min(chunkE, (long)nElems - e0);
@@ -665,8 +670,12 @@ int galx_set_mesh(galx_ctx* c, int nElems, int nSides, const int* E2S, const int
                 }
                 o[9 * n3 + x] = sJ[id];
             }
+            double* so = sbuf.data() + e * 10 * ns;
+            for (int q = 0; q < 10; q++)
+                for (int t = 0; t < ns; t++) so[q * ns + t] = o[q * n3 + snode[t]];
         }
         HIPCHK(hipMemcpy(c->dGeo + e0 * 10 * n3, buf.data(), ne * 10 * n3 * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->dSgeo + e0 * 10 * ns, sbuf.data(), ne * 10 * ns * sizeof(double), hipMemcpyHostToDevice));
     }
     const long chunkS = 16384;
     buf.resize(10 * chunkS * n2);

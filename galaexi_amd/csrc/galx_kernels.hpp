@@ -43,7 +43,15 @@
 #endif
 // k_lift: gradients of the surface nodes only (0 off, 1 FP64 form only, 2 both forms)
 #ifndef GALX_LIFT_SURF
-#define GALX_LIFT_SURF 1
+#define GALX_LIFT_SURF 2
+#endif
+// k_lift, surface-node gradients: the metrics read from the surface-node copy of the geometry (1 on)
+#ifndef GALX_SURF_GEO
+#define GALX_SURF_GEO 1
+#endif
+// k_rhs, split form: each pair of an i- or j-line formed once and shared between the two lanes (1 on)
+#ifndef GALX_PAIR_SPLIT
+#define GALX_PAIR_SPLIT 1
 #endif
 // k_rhs, mixed split form: the nine metric terms of every node stay in LDS (FP32) from the start, so the volume
 // terms read them there instead of from memory a second time (1 on)
@@ -59,6 +67,9 @@ struct Dm {
     static constexpr int block = n3 <= 64 ? 64 : ((n3 + 63) / 64) * 64;
     static constexpr int ft = (nf + block - 1) / block;
 };
+
+// nodes on the element surface, n^3 - (n-2)^3 of them
+__host__ __device__ constexpr int surface_count(int N) { return 2 * (N + 1) * (N + 1) + (N - 1) * 4 * N; }
 
 // LDS layout of the per-node arrays read along element lines (lifting, split and viscous volume terms) and of
 // the D-matrices: node (i,j,k) at i + np*(j + n*k) with row stride np = n + GALX_LDS_PAD.  A padded stride (n+1)
@@ -132,6 +143,7 @@ struct Mesh {
     const double* fgeo;
     const int* conn;
     const int* inv;
+    const double* sgeo;  // [nElems][10][surface_count(N)]: geo at the surface nodes, in surface_node order
 };
 
 struct Stage {
@@ -248,6 +260,17 @@ __device__ __forceinline__ void load_node_geo(int e, int node, const Mesh& mesh,
     for (int q = 0; q < 10; q++) gm[q] = T(ge[q * n3]);
 }
 
+// the same ten values of surface node t (node surface_node<N>(t)) from the surface-node copy: 10 * surface_count
+// contiguous doubles per element instead of every 64-byte row of the element's volume planes
+template <int N, typename T>
+__device__ __forceinline__ void load_surf_geo(int e, int t, const Mesh& mesh, T* gm)
+{
+    constexpr int ns = surface_count(N);
+    const double* ge = mesh.sgeo + (long)e * 10 * ns + t;
+#pragma unroll
+    for (int q = 0; q < 10; q++) gm[q] = T(ge[q * ns]);
+}
+
 // gm: the node's Ja^1(3), Ja^2(3), Ja^3(3), sJ (load_node_geo), loaded by the caller ahead of the barrier that
 // precedes this call so that their latency overlaps the face tasks
 template <int N, typename T, class MT>
@@ -308,16 +331,17 @@ __device__ __forceinline__ void node_gradient(int e, int node, const Mesh& mesh,
     node_gradient<N, T>(node, M, s_lift, s_lflux, LHm, LHp, gm, g);
 }
 
-// nodes on the element surface, n^3 - (n-2)^3 of them: the k=0 plane, the rings of the inner k planes, the k=N plane
+// the k=0 plane, the rings of the inner k planes, the k=N plane
 template <int N>
 struct Surf {
-    static constexpr int n = N + 1, ring = 4 * N, count = 2 * n * n + (n - 2) * 4 * N;
+    static constexpr int n = N + 1, ring = 4 * N, count = surface_count(N);
 };
 
-template <int N>
-__host__ __device__ __forceinline__ int surface_node(int t)
+// volume node (i + n*(j + n*k)) of surface node t, 0 <= t < surface_count(N); the host builds the surface-node
+// metric array in this order
+__host__ __device__ __forceinline__ int surface_node_of(int N, int t)
 {
-    constexpr int n = N + 1, n2 = n * n, ring = Surf<N>::ring;
+    const int n = N + 1, n2 = n * n, ring = 4 * N;
     if (t < n2) return t;
     t -= n2;
     if (t < (n - 2) * ring) {
@@ -339,6 +363,12 @@ __host__ __device__ __forceinline__ int surface_node(int t)
         return i + n * (j + n * k);
     }
     return t - (n - 2) * ring + n2 * N;
+}
+
+template <int N>
+__host__ __device__ __forceinline__ int surface_node(int t)
+{
+    return surface_node_of(N, t);
 }
 
 template <int N, typename T, typename TT, bool TR>
@@ -468,7 +498,12 @@ __global__ void __launch_bounds__(Dm<N>::block, (LiftWaves<N, T>::value))
     const bool gnode = surf ? tid < Surf<N>::count : tid < n3;
     const int gn = surf ? (gnode ? surface_node<N>(tid) : 0) : tid;
     T gm[10];
-    if (gnode) load_node_geo<N, T>(e, gn, mesh, gm);
+    if (gnode) {
+        if (surf && GALX_SURF_GEO)
+            load_surf_geo<N, T>(e, tid, mesh, gm);
+        else
+            load_node_geo<N, T>(e, gn, mesh, gm);
+    }
     if (tid < n3) {
         load_state<N, T>(e, tid, U, gd, uc, pr, H);
         const int pt = Lay<N>::at(tid);
@@ -617,17 +652,25 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                              (GALX_PREFETCH_GEO == 2 || GALX_PREFETCH_GEO == 1);
     constexpr int w_c = SPLIT ? (geo_lds ? 10 : 13) * n3p : 5 * n3p;  // volume fluxes / split-form partner data
     constexpr int w_d = 6 * 5 * n2;                   // face results, new traces
+    // split form: the i- and j-lines lie inside one 64-lane wave (n and n^2 divide 64: N = 1, 3, 7), so the pairs of
+    // those lines are shared between lanes instead of formed twice
+    constexpr bool pair_lines = GALX_PAIR_SPLIT && SPLIT && 64 % n == 0 && 64 % n2 == 0;
     constexpr int w_ac = w_a > w_c ? w_a : w_c;
     constexpr int lds_w = w_ac > w_d ? w_ac : w_d;
     __shared__ T s_w[lds_w];
     __shared__ T s_geo[geo_lds ? 9 * n3p : 1];  // [3 directions][3][n3p] metrics Ja^1, Ja^2, Ja^3
     __shared__ T s_nb[6 * 5 * n2];  // neighbour trace records, from the lifting to the face tasks
     __shared__ Mats<N, T> M;
+    __shared__ T s_dp[pair_lines ? n2 : 1];  // DVolSurf by cyclic shift for the shared pairs
     const int e = blockIdx.x, tid = threadIdx.x;
     const bool active = tid < n3;
     const int i = tid % n, j = (tid / n) % n, k = tid / n2;
     const int pt = Lay<N>::at(tid);
     load_mats<N, T>(M, mats, tid, D::block);
+    if (pair_lines && tid < n2) {
+        const int sh = tid / n, a = tid % n;
+        s_dp[tid] = mats[2 * n2 + ((a + sh) & (n - 1)) + n * a];
+    }
     const Gas<T> gas = make_gas<T>(gd);
     T uc[5], pr[6], Hme = T(0);
     // the node's metrics, issued with the state so that the lifting does not wait for them (measured on MI355X at
@@ -736,6 +779,42 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll
                     for (int v = 1; v < 5; v++) ut[v] = ut[v] + dh * s_f[(v - 1) * n3p + o];
                 }
+                if (pair_lines && d < 2) {
+                    // the two-point flux is symmetric, G(a,b) = G(b,a) bit for bit (sums and products of the
+                    // same operands), so each pair of a line is formed once: the lane at line position a forms
+                    // the pairs (a, a+s) for s = 1..(n-1)/2 (mod n) and reads the pair (a-s, a) from the lane s
+                    // positions back with a cross-lane permute; for even n both ends of a pair (a, a+n/2) form
+                    // it.  The diagonal, DVolSurf(a,a) = 0 up to rounding for Gauss-Lobatto (dg.f90:229-233),
+                    // is left out.  About half the flux evaluations of the loop over all l below.
+                    constexpr int msk = n - 1;                  // n is a power of two here
+                    const int st = d == 0 ? 1 : n;              // lane distance between neighbours on the line
+                    const int lane0 = (tid & 63) - idx * st;    // lane of line position 0
+                    const T* dp = s_dp + idx;                   // s_dp[s*n + a] = DVolSurf((a+s) mod n, a)
+#pragma unroll
+                    for (int s = 1; s <= n / 2; s++) {
+                        const int b = (idx + s) & msk;          // partner formed here
+                        const int o = ln(b);
+                        T Mo[3] = {s_m[o], s_m[n3p + o], s_m[2 * n3p + o]};
+                        T G[5];
+                        split_volume_flux_pi_half<T>(hv[0], hv[1], hv[2], hv[3], hv[4], hv[5], s_v[o],
+                                                     s_v[n3p + o], s_v[2 * n3p + o], s_v[3 * n3p + o],
+                                                     s_v[4 * n3p + o], s_v[5 * n3p + o], hm, Mo, G);
+                        const T db = dp[s * n];
+                        if (2 * s < n) {
+                            const int src = lane0 + ((idx - s) & msk) * st;  // the lane that formed (a-s, a)
+                            const T da = dp[(n - s) * n];
+#pragma unroll
+                            for (int v = 0; v < 5; v++) {
+                                const T r = __shfl(G[v], src);
+                                tmp[v] = tmp[v] + db * G[v];
+                                tmp[v] = tmp[v] + da * r;
+                            }
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < 5; v++) tmp[v] = tmp[v] + db * G[v];
+                        }
+                    }
+                } else {
                 // all l, the diagonal included as in VolInt_splitForm_Kernel (volint.f90:307-317): DVolSurf(i,i)
                 // vanishes for Gauss-Lobatto up to rounding (dg.f90:229-233), and a uniform loop keeps the
                 // wave free of per-lane branches
@@ -750,6 +829,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                     T dv = M.dvs(l, idx);
 #pragma unroll
                     for (int v = 0; v < 5; v++) tmp[v] = tmp[v] + dv * G[v];
+                }
                 }
             }
             __syncthreads();
