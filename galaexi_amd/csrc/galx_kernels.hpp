@@ -37,7 +37,8 @@
 #ifndef GALX_LDS_SWZ
 #define GALX_LDS_SWZ 0
 #endif
-// k_rhs: the node metrics issued with the state, ahead of the lifting (0 off, 1 mixed form only, 2 both forms)
+// k_rhs: the node metrics issued with the state, ahead of the lifting (0 off, 1 mixed form and, with GALX_GEO_REG,
+// the FP64 split form, 2 every form)
 #ifndef GALX_PREFETCH_GEO
 #define GALX_PREFETCH_GEO 1
 #endif
@@ -48,6 +49,11 @@
 // k_lift, surface-node gradients: the metrics read from the surface-node copy of the geometry (1 on)
 #ifndef GALX_SURF_GEO
 #define GALX_SURF_GEO 1
+#endif
+// k_rhs, FP64 split form: the metrics and sJ loaded with the state stay in registers for the volume terms and the
+// update instead of being read again (1 on; measured on MI355X at N=7: 51 B/DOF fewer HBM reads, 7 % faster)
+#ifndef GALX_GEO_REG
+#define GALX_GEO_REG 1
 #endif
 // k_rhs, split form: each pair of an i- or j-line formed once and shared between the two lanes (1 on)
 #ifndef GALX_PAIR_SPLIT
@@ -674,8 +680,10 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     const Gas<T> gas = make_gas<T>(gd);
     T uc[5], pr[6], Hme = T(0);
     // the node's metrics, issued with the state so that the lifting does not wait for them (measured on MI355X at
-    // N=7: 8 % faster in the mixed form, 0.8 % slower in FP64)
-    constexpr bool pgeo = GALX_PREFETCH_GEO == 2 || (GALX_PREFETCH_GEO == 1 && sizeof(T) == 4);
+    // N=7: 8 % faster in the mixed form; 0.8 % slower in FP64 as a prefetch alone, 7 % faster once the FP64 split
+    // form also keeps them for the volume terms and the update, GALX_GEO_REG)
+    constexpr bool pgeo =
+        GALX_PREFETCH_GEO == 2 || (GALX_PREFETCH_GEO == 1 && (sizeof(T) == 4 || (SPLIT && GALX_GEO_REG)));
     T gm[10];
     if (active) {
         if (pgeo) load_node_geo<N, T>(e, tid, mesh, gm);
@@ -758,7 +766,10 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                 } else {
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
-                        mme[c] = T(ge[(3 * d + c) * n3]);
+                        if (pgeo && GALX_GEO_REG)  // the prefetched values, kept in registers since the load
+                            mme[c] = d == 0 ? gm[c] : (d == 1 ? gm[3 + c] : gm[6 + c]);
+                        else
+                            mme[c] = T(ge[(3 * d + c) * n3]);
                         hm[c] = T(0.5) * mme[c];
                         s_m[c * n3p + pt] = hm[c];
                     }
@@ -890,7 +901,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         double* ut_out = (st.mode == 0 ? Ut : Kreg) + (long)e * 5 * n3 + tid;
         double* un = Uout + (long)e * 5 * n3 + tid;
         double kv[5] = {0, 0, 0, 0, 0}, uv[5] = {0, 0, 0, 0, 0};
-        const double sJd = ge[9 * n3];
+        const double sJd = (pgeo && GALX_GEO_REG && sizeof(T) == 8) ? double(gm[9]) : ge[9 * n3];
         if (st.mode != 0) {
             const double* uo = U + (long)e * 5 * n3 + tid;
 #pragma unroll
