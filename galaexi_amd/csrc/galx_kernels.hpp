@@ -55,6 +55,12 @@
 #ifndef GALX_GEO_REG
 #define GALX_GEO_REG 1
 #endif
+// k_rhs, FP64: the own trace records of the face nodes from the node threads through LDS (1 on).  Measured on
+// MI355X at N=7: 1.66-1.68 instead of 1.57-1.58 ms, as keeping the state live through the volume terms spills
+// 20 bytes per lane; off by default.
+#ifndef GALX_OWN_LDS
+#define GALX_OWN_LDS 0
+#endif
 // k_rhs, split form: each pair of an i- or j-line formed once and shared between the two lanes (1 on)
 #ifndef GALX_PAIR_SPLIT
 #define GALX_PAIR_SPLIT 1
@@ -657,7 +663,10 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     constexpr bool geo_lds = GALX_GEO_LDS && SPLIT && sizeof(T) == 4 &&
                              (GALX_PREFETCH_GEO == 2 || GALX_PREFETCH_GEO == 1);
     constexpr int w_c = SPLIT ? (geo_lds ? 10 : 13) * n3p : 5 * n3p;  // volume fluxes / split-form partner data
-    constexpr int w_d = 6 * 5 * n2;                   // face results, new traces
+    // FP64: the own trace records of the face nodes reach the face tasks through LDS, formed by the node threads
+    // from the state they hold (uc is the FP64 state itself there), instead of a second gather of U
+    constexpr bool own_lds = GALX_OWN_LDS && sizeof(T) == 8;
+    constexpr int w_d = (own_lds ? 2 : 1) * 6 * 5 * n2;  // face results (and own records), new traces
     // split form: the i- and j-lines lie inside one 64-lane wave (n and n^2 divide 64: N = 1, 3, 7), so the pairs of
     // those lines are shared between lanes instead of formed twice
     constexpr bool pair_lines = GALX_PAIR_SPLIT && SPLIT && 64 % n == 0 && 64 % n2 == 0;
@@ -851,6 +860,22 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 
     // ---- face fluxes (FillFlux) and surface integral (SurfInt_Device) ----
     T* s_res = s_w;  // [6][5][n2]
+    T* s_own = s_w + 6 * 5 * n2;  // [6][5][n2] own trace records (own_lds)
+    if (own_lds) {
+        // the volume terms ended with a barrier: s_w is free
+        if (active) {
+            double c[5] = {double(uc[0]), double(uc[1]), double(uc[2]), double(uc[3]), double(uc[4])}, tr[5];
+            trace_record<false>(c, gd.kappa, gd.pref, tr);
+#pragma unroll
+            for (int ls = 1; ls <= 6; ls++) {
+                int ab;
+                if (!on_face<N>(ls, i, j, k, ab)) continue;
+#pragma unroll
+                for (int v = 0; v < 5; v++) s_own[(ls - 1) * 5 * n2 + v * n2 + ab] = T(tr[v]);
+            }
+        }
+        __syncthreads();
+    }
     for (int t = tid; t < D::nf; t += D::block) {
         int ls = t / n2 + 1, ab = t - (ls - 1) * n2, a = ab % n, b = ab / n;
         int node = face_vol_node<N>(ls, a, b);
@@ -859,9 +884,12 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         int flip = c & 7;
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
         T Uo[5], fvo[4], Un[5], fvn[4];
-        {
-            // from U at the face node (in L2 since the state load), not from the own slot of the trace buffer:
-            // the same record, but that read goes to memory (measured 1.5 % slower in FP64 on MI355X)
+        if (own_lds) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) Uo[v] = s_own[(ls - 1) * 5 * n2 + v * n2 + ab];
+        } else {
+            // from U at the face node, not from the own slot of the trace buffer: the same record, but that read
+            // goes to memory (measured 1.5 % slower in FP64 on MI355X)
             const double* uo = U + (long)e * 5 * n3 + node;
             double c[5], tr[5];
 #pragma unroll
