@@ -3,6 +3,7 @@
 // Data layout in HBM (one rank's element share on one GPU):
 //   U, Ut, Ut_tmp : double [nElems][5][n^3]          (element-major, variable planes, i fastest)
 //   geo           : double [nElems][10][n^3]         (Ja^1(3), Ja^2(3), Ja^3(3), sJ)
+//   sgeo          : double [nElems][10][surface_count(N)] geo at the surface nodes only (k_lift)
 //   fgeo          : double [nSides][10][n^2]         (NormVec(3), TangVec1(3), TangVec2(3), SurfElem), master (p,q)
 //   utr[2]        : TT [nSides][2][5][n^2]           U traces, slot 0 master / 1 slave, master (p,q); double buffered
 //   vtr           : TT [nSides][2][4][n^2]           viscous normal-flux traces F_v(U,grad U).n_master, components
