@@ -506,7 +506,9 @@ __global__ void __launch_bounds__(Dm<N>::block, (LiftWaves<N, T>::value))
     T uc[5], pr[6], H;
     // only the surface nodes' gradients are used (for F_v.n on the faces): with `surf` the first Surf<N>::count
     // threads evaluate them, one surface node each, and the interior nodes' gradients and metrics are skipped
-    // (measured on MI355X at N=7: 1.8 % faster in FP64, 3 % slower in the mixed form)
+    // (measured on MI355X at N=7 with the metrics from the surface-node copy: 8 % faster in FP64, 12 % in the
+    // mixed form; with the metrics from the volume planes, whose 64-byte rows all hold surface nodes, 1.8 % faster
+    // in FP64 and 3 % slower in the mixed form)
     constexpr bool surf = GALX_LIFT_SURF == 2 || (GALX_LIFT_SURF == 1 && sizeof(T) == 8);
     const bool gnode = surf ? tid < Surf<N>::count : tid < n3;
     const int gn = surf ? (gnode ? surface_node<N>(tid) : 0) : tid;
