@@ -17,13 +17,6 @@
 #pragma once
 #include "galx_kernels.hpp"
 
-// k_adj_main: the node's metrics and sJ read once into registers for its three phases instead of once per phase
-// (1 on).  Measured on MI355X at N=7: the VJP takes 4.64 instead of 3.96 ms, as the kernel, already at 128 VGPRs,
-// spills 172 instead of 76 bytes per lane; off by default.
-#ifndef GALX_ADJ_GEO_REG
-#define GALX_ADJ_GEO_REG 0
-#endif
-
 namespace galx {
 
 // adjoint with respect to (rho,u,v,w,p,H,T) and a direct E dependence
@@ -381,19 +374,9 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
     const Gas<double> gas = make_gas<double>(gd);
     double pr[6] = {1, 0, 0, 0, 1, 1};
     const double* ge = mesh.geo + (long)e * 10 * n3 + tid;
-#if GALX_ADJ_GEO_REG
-    // the node's metrics and sJ, read once for the three phases that use them
-    double gm[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (active) {
-#pragma unroll
-        for (int q = 0; q < 10; q++) gm[q] = ge[q * n3];
-    }
-#define GALX_ADJ_GD(d, c) ((d) == 0 ? gm[c] : ((d) == 1 ? gm[3 + (c)] : gm[6 + (c)]))
-#define GALX_ADJ_SJ gm[9]
-#else
-#define GALX_ADJ_GD(d, c) ge[(3 * (d) + (c)) * n3]
-#define GALX_ADJ_SJ ge[9 * n3]
-#endif
+    // the node's metrics and sJ are read in each of the three phases that use them: read once into registers
+    // instead, they made the VJP 17 % slower at N=7 on MI355X (the kernel, at 128 VGPRs, spilled 172 instead of 76
+    // bytes per lane)
     PrimBar pb;
     prim_bar_zero(pb);
     // ---- adjoint of the split volume flux (VolInt_splitForm_Kernel): X(i) += sum_l DVS(l,i) F#(i,l) with F#
@@ -411,7 +394,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
             s_v[3 * n3 + tid] = pr[3];
             s_v[4 * n3 + tid] = pr[4];
             s_v[5 * n3 + tid] = Hme;
-            double msJ = -GALX_ADJ_SJ;
+            double msJ = -ge[9 * n3];
             const double* l = lam + (long)e * 5 * n3 + tid;
 #pragma unroll
             for (int v = 0; v < 5; v++) s_mu[v * n3 + tid] = msJ * l[v * n3];
@@ -421,7 +404,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
             double mme[3] = {0, 0, 0};
             if (active) {
 #pragma unroll
-                for (int c = 0; c < 3; c++) mme[c] = GALX_ADJ_GD(d, c);
+                for (int c = 0; c < 3; c++) mme[c] = ge[(3 * (d) + (c)) * n3];
                 s_m[tid] = mme[0];
                 s_m[n3 + tid] = mme[1];
                 s_m[2 * n3 + tid] = mme[2];
@@ -496,7 +479,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
 #pragma unroll
                     for (int v = 1; v < 5; v++) fb[v] += dh * s_mu[v * n3 + base + m * stride];
                 }
-                double md[3] = {GALX_ADJ_GD(d, 0), GALX_ADJ_GD(d, 1), GALX_ADJ_GD(d, 2)};
+                double md[3] = {ge[(3 * d + 0) * n3], ge[(3 * d + 1) * n3], ge[(3 * d + 2) * n3]};
                 visc_tau_adjoint(tau, md, fb[4], uvw);
                 visc_grad_adjoint(pr[1], pr[2], pr[3], md, fb[1], fb[2], fb[3], fb[4], gb);
             }
@@ -580,7 +563,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
         }
         // ghat = sJ*Gbar: adjoint of g = sJ*(...) in lift_gradient
         visc_grad_expand(gb, gd.mu, gd.lambda, Gb);
-        double sJ = GALX_ADJ_SJ;
+        double sJ = ge[9 * n3];
 #pragma unroll
         for (int q = 0; q < 12; q++) Gb[q] *= sJ;
         // BR1 surface term (Lifting_FillFlux lifting_fillflux.t90:39-93): g_own += sJ L_Hat SurfElem/2 (q_slave -
@@ -615,7 +598,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
 #pragma unroll 1
     for (int d = 0; d < 3; d++) {
         if (active) {
-            const double m0 = GALX_ADJ_GD(d, 0), m1 = GALX_ADJ_GD(d, 1), m2 = GALX_ADJ_GD(d, 2);
+            const double m0 = ge[(3 * d + 0) * n3], m1 = ge[(3 * d + 1) * n3], m2 = ge[(3 * d + 2) * n3];
 #pragma unroll
             for (int q = 0; q < 4; q++) s_a[q * n3 + tid] = m0 * Gb[q] + m1 * Gb[4 + q] + m2 * Gb[8 + q];
         }
@@ -652,8 +635,6 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
         if (kscale) lp[v * n3] = a * lp[v * n3];
     }
 }
-#undef GALX_ADJ_GD
-#undef GALX_ADJ_SJ
 
 // ---------------------------------------------------------------------------------------------------------
 // k_adj_face: the neighbour's BR1 surface term of a VJP on its own (the last one of a sweep, or a single VJP)

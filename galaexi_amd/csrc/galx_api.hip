@@ -479,6 +479,169 @@ static GasD gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0
 // the adjoint kernels and their FP64 trace records use the pressure itself (no offset)
 static GasD adj_gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, 0.0}; }
 
+// two pinned host buffers that alternate as staging for host-to-device copies: next() returns a buffer whose
+// previous copy has finished, issued() marks the copies just enqueued from it
+struct Staging {
+    double* p[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool busy[2] = {false, false};
+    int k = 1;
+    int init(size_t bytes)
+    {
+        for (int i = 0; i < 2; i++) {
+            hipError_t e = hipHostMalloc((void**)&p[i], bytes, hipHostMallocDefault);
+            if (e != hipSuccess) {
+                p[i] = nullptr;
+                return fail(GALX_E_NOMEM, "hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+            }
+            e = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming);
+            if (e != hipSuccess) {
+                ev[i] = nullptr;
+                return fail(GALX_E_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+            }
+        }
+        return GALX_OK;
+    }
+    double* next()
+    {
+        k ^= 1;
+        if (busy[k]) {
+            hipError_t e = hipEventSynchronize(ev[k]);
+            if (e != hipSuccess) {
+                fail(GALX_E_HIP, "hipEventSynchronize: %s", hipGetErrorString(e));
+                return nullptr;
+            }
+            busy[k] = false;
+        }
+        return p[k];
+    }
+    int issued(hipStream_t s)
+    {
+        HIPCHK(hipEventRecord(ev[k], s));
+        busy[k] = true;
+        return GALX_OK;
+    }
+    ~Staging()
+    {
+        for (int i = 0; i < 2; i++) {
+            if (ev[i]) {
+                (void)hipEventSynchronize(ev[i]);
+                (void)hipEventDestroy(ev[i]);
+            }
+            if (p[i]) (void)hipHostFree(p[i]);
+        }
+    }
+};
+
+// the buffers galx_set_mesh allocates
+static void free_mesh(galx_ctx* c)
+{
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    double** pd[] = {&c->dU, &c->dK, &c->dGeo, &c->dFgeo, &c->dSgeo, &c->dDtc, &c->dDtv, &c->dDtOut};
+    for (double** p : pd) {
+        dfree(*p);
+        *p = nullptr;
+    }
+    int** pi[] = {&c->dConn, &c->dInv, &c->dErr};
+    for (int** p : pi) {
+        dfree(*p);
+        *p = nullptr;
+    }
+    void** pv[] = {&c->dUtr[0], &c->dUtr[1], &c->dVtr};
+    for (void** p : pv) {
+        dfree(*p);
+        *p = nullptr;
+    }
+}
+
+// device buffers of galx_set_mesh and their contents; on failure the caller frees what was allocated
+static int set_mesh_device(galx_ctx* c, int nElems, int nSides, const std::vector<int>& conn,
+                           const std::vector<int>& inv, const double* Mf, const double* Mg, const double* Mh,
+                           const double* sJ, const double* nv, const double* t1, const double* t2, const double* se)
+{
+    const int n2 = c->n2, n3 = c->n3;
+    int rc;
+    long nDOF = (long)nElems * n3, nfn = (long)nSides * n2;
+    c->nElems = nElems;
+    c->nSides = nSides;
+    if ((rc = dalloc(c, &c->dU, 5 * nDOF * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dK, 5 * nDOF * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dGeo, 10 * nDOF * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dFgeo, 10 * nfn * sizeof(double)))) return rc;
+    const int ns = surface_count(c->cfg.N);
+    if ((rc = dalloc(c, &c->dSgeo, 10L * nElems * ns * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dConn, 6L * nElems * sizeof(int)))) return rc;
+    if ((rc = dalloc(c, &c->dInv, inv.size() * sizeof(int)))) return rc;
+    if ((rc = dalloc(c, (char**)&c->dUtr[0], 2 * 5 * nfn * c->tsz))) return rc;
+    if ((rc = dalloc(c, (char**)&c->dUtr[1], 2 * 5 * nfn * c->tsz))) return rc;
+    if ((rc = dalloc(c, (char**)&c->dVtr, 2 * 4 * nfn * c->tsz))) return rc;
+    if ((rc = dalloc(c, &c->dDtc, nElems * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dDtv, nElems * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dDtOut, 4 * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dErr, sizeof(int)))) return rc;
+    HIPCHK(hipMemcpy(c->dConn, conn.data(), conn.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->dInv, inv.data(), inv.size() * sizeof(int), hipMemcpyHostToDevice));
+    // geometry into element / side planes, in chunks through two pinned staging buffers: the host fills one while
+    // the copy of the other runs.  The surface-node copy of the element geometry (k_lift's metrics) comes from the
+    // same planes.
+    const long chunkE = 1024, chunkS = 4096;  // 66 MB per staging buffer at N=7
+    const size_t stageB = std::max(10 * chunkE * (n3 + ns), 10 * chunkS * n2) * sizeof(double);
+    Staging stg;
+    if ((rc = stg.init(stageB))) return rc;
+    std::vector<int> snode(ns);
+    for (int t = 0; t < ns; t++) snode[t] = surface_node_of(c->cfg.N, t);
+    for (long e0 = 0; e0 < nElems; e0 += chunkE) {
+        long ne = std::min(chunkE, (long)nElems - e0);
+        double* buf = stg.next();
+        if (!buf) return fail(GALX_E_HIP, "staging: %s", g_err.c_str());
+        double* sbuf = buf + 10 * chunkE * n3;
+        for (long e = 0; e < ne; e++) {
+            double* o = buf + e * 10 * n3;
+            long ge = e0 + e;
+            for (int x = 0; x < n3; x++) {
+                long id = ge * n3 + x;
+                for (int d = 0; d < 3; d++) {
+                    o[d * n3 + x] = Mf[3 * id + d];
+                    o[(3 + d) * n3 + x] = Mg[3 * id + d];
+                    o[(6 + d) * n3 + x] = Mh[3 * id + d];
+                }
+                o[9 * n3 + x] = sJ[id];
+            }
+            double* so = sbuf + e * 10 * ns;
+            for (int q = 0; q < 10; q++)
+                for (int t = 0; t < ns; t++) so[q * ns + t] = o[q * n3 + snode[t]];
+        }
+        HIPCHK(hipMemcpyAsync(c->dGeo + e0 * 10 * n3, buf, ne * 10 * n3 * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(c->dSgeo + e0 * 10 * ns, sbuf, ne * 10 * ns * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        if ((rc = stg.issued(c->stream))) return rc;
+    }
+    for (long s0 = 0; s0 < nSides; s0 += chunkS) {
+        long nsc = std::min(chunkS, (long)nSides - s0);
+        double* buf = stg.next();
+        if (!buf) return fail(GALX_E_HIP, "staging: %s", g_err.c_str());
+        for (long s = 0; s < nsc; s++) {
+            double* o = buf + s * 10 * n2;
+            long gs = s0 + s;
+            for (int x = 0; x < n2; x++) {
+                long id = gs * n2 + x;
+                for (int d = 0; d < 3; d++) {
+                    o[d * n2 + x] = nv[3 * id + d];
+                    o[(3 + d) * n2 + x] = t1[3 * id + d];
+                    o[(6 + d) * n2 + x] = t2[3 * id + d];
+                }
+                o[9 * n2 + x] = se[id];
+            }
+        }
+        HIPCHK(hipMemcpyAsync(c->dFgeo + s0 * 10 * n2, buf, nsc * 10 * n2 * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+        if ((rc = stg.issued(c->stream))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
 extern "C" {
 
 const char* galx_version(void) { return "galx 0.1 (gfx950)"; }
@@ -628,80 +791,24 @@ int galx_set_mesh(galx_ctx* c, int nElems, int nSides, const int* E2S, const int
             if (seen[2L * (side - 1) + (flip ? 1 : 0)]++) return fail(GALX_E_MESH, "side %d visited twice", side);
             conn[6L * e + ls - 1] = ((side - 1) << 3) | flip;
         }
-    int rc;
-    long nDOF = (long)nElems * n3, nfn = (long)nSides * n2;
-    c->nElems = nElems;
-    c->nSides = nSides;
-    if ((rc = dalloc(c, &c->dU, 5 * nDOF * sizeof(double)))) return rc;
-    if ((rc = dalloc(c, &c->dK, 5 * nDOF * sizeof(double)))) return rc;
-    if ((rc = dalloc(c, &c->dGeo, 10 * nDOF * sizeof(double)))) return rc;
-    if ((rc = dalloc(c, &c->dFgeo, 10 * nfn * sizeof(double)))) return rc;
-    const int ns = surface_count(c->cfg.N);
-    if ((rc = dalloc(c, &c->dSgeo, 10L * nElems * ns * sizeof(double)))) return rc;
-    if ((rc = dalloc(c, &c->dConn, 6L * nElems * sizeof(int)))) return rc;
-    if ((rc = dalloc(c, &c->dInv, inv.size() * sizeof(int)))) return rc;
-    if ((rc = dalloc(c, (char**)&c->dUtr[0], 2 * 5 * nfn * c->tsz))) return rc;
-    if ((rc = dalloc(c, (char**)&c->dUtr[1], 2 * 5 * nfn * c->tsz))) return rc;
-    if ((rc = dalloc(c, (char**)&c->dVtr, 2 * 4 * nfn * c->tsz))) return rc;
-    if ((rc = dalloc(c, &c->dDtc, nElems * sizeof(double)))) return rc;
-    if ((rc = dalloc(c, &c->dDtv, nElems * sizeof(double)))) return rc;
-    if ((rc = dalloc(c, &c->dDtOut, 4 * sizeof(double)))) return rc;
-    if ((rc = dalloc(c, &c->dErr, sizeof(int)))) return rc;
-    HIPCHK(hipMemcpy(c->dConn, conn.data(), conn.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->dInv, inv.data(), inv.size() * sizeof(int), hipMemcpyHostToDevice));
-    // geometry into element / side planes, in chunks through pinned staging; the surface-node copy of the element
-    // geometry (k_lift's metrics) from the same planes
-    const long chunkE = 4096;
-    std::vector<double> buf, sbuf(10 * chunkE * ns);
-    std::vector<int> snode(ns);
-    for (int t = 0; t < ns; t++) snode[t] = surface_node_of(c->cfg.N, t);
-    buf.resize(10 * chunkE * n3);
-    for (long e0 = 0; e0 < nElems; e0 += chunkE) {
-        long ne = std::min(chunkE, (long)nElems - e0);
-        for (long e = 0; e < ne; e++) {
-            double* o = buf.data() + e * 10 * n3;
-            long ge = e0 + e;
-            for (int x = 0; x < n3; x++) {
-                long id = ge * n3 + x;
-                for (int d = 0; d < 3; d++) {
-                    o[d * n3 + x] = Mf[3 * id + d];
-                    o[(3 + d) * n3 + x] = Mg[3 * id + d];
-                    o[(6 + d) * n3 + x] = Mh[3 * id + d];
-                }
-                o[9 * n3 + x] = sJ[id];
-            }
-            double* so = sbuf.data() + e * 10 * ns;
-            for (int q = 0; q < 10; q++)
-                for (int t = 0; t < ns; t++) so[q * ns + t] = o[q * n3 + snode[t]];
-        }
-        HIPCHK(hipMemcpy(c->dGeo + e0 * 10 * n3, buf.data(), ne * 10 * n3 * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->dSgeo + e0 * 10 * ns, sbuf.data(), ne * 10 * ns * sizeof(double), hipMemcpyHostToDevice));
-    }
-    const long chunkS = 16384;
-    buf.resize(10 * chunkS * n2);
-    for (long s0 = 0; s0 < nSides; s0 += chunkS) {
-        long ns = std::min(chunkS, (long)nSides - s0);
-        for (long s = 0; s < ns; s++) {
-            double* o = buf.data() + s * 10 * n2;
-            long gs = s0 + s;
-            for (int x = 0; x < n2; x++) {
-                long id = gs * n2 + x;
-                for (int d = 0; d < 3; d++) {
-                    o[d * n2 + x] = nv[3 * id + d];
-                    o[(3 + d) * n2 + x] = t1[3 * id + d];
-                    o[(6 + d) * n2 + x] = t2[3 * id + d];
-                }
-                o[9 * n2 + x] = se[id];
-            }
-        }
-        HIPCHK(hipMemcpy(c->dFgeo + s0 * 10 * n2, buf.data(), ns * 10 * n2 * sizeof(double), hipMemcpyHostToDevice));
+    const long long bytes0 = c->bytes;
+    int rc = set_mesh_device(c, nElems, nSides, conn, inv, Mf, Mg, Mh, sJ, nv, t1, t2, se);
+    if (rc) {
+        // nothing of a failed call stays allocated, so that a later call sizes every buffer for its own mesh
+        std::string msg = g_err;
+        free_mesh(c);
+        c->bytes = bytes0;
+        c->nElems = c->nSides = 0;
+        return fail(rc, "%s", msg.c_str());
     }
     c->meshSet = true;
     return GALX_OK;
 }
 
+
 static int after_state(galx_ctx* c)
 {
+    c->trajValid = false;  // a recorded trajectory no longer ends at the context state
     HIPCHK(hipMemsetAsync(c->dK, 0, 5L * c->nElems * c->n3 * sizeof(double), c->stream));
     c->cur = 0;
     c->pref = 0.0;
@@ -750,6 +857,7 @@ int galx_set_source(galx_ctx* c, int exactFunc, const double* advVel, const doub
 {
     if (!c) return fail(GALX_E_ARG, "null context");
     if (!c->meshSet) return fail(GALX_E_STATE, "set_mesh first");
+    c->trajValid = false;  // a recorded trajectory's recompute would see another operator
     if (exactFunc != 4 && exactFunc != 41 && exactFunc != 42 && exactFunc != 43) {
         c->srcFunc = 0;
         return GALX_OK;
@@ -966,6 +1074,7 @@ int galx_lserk_steps(galx_ctx* c, double t, double dt, int nSteps)
     if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
     if (nSteps < 0 || !(dt == dt)) return fail(GALX_E_ARG, "bad step arguments");
     HIPCHK(hipSetDevice(c->cfg.device));
+    c->trajValid = false;  // the state moves on from the recorded trajectory's final state
     int rc = lserk_enqueue(c, t, dt, 0, nSteps);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -980,6 +1089,7 @@ int galx_lserk_stage(galx_ctx* c, double t, double dt, int iStage)
     if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
     if (iStage < 1 || iStage > 5 || !(dt == dt)) return fail(GALX_E_ARG, "bad stage arguments");
     HIPCHK(hipSetDevice(c->cfg.device));
+    c->trajValid = false;
     int stage = iStage - 1;
     launch_lift(c);
     double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];

@@ -29,48 +29,6 @@
 #ifndef GALX_MIXED_WAVES
 #define GALX_MIXED_WAVES 6
 #endif
-#ifndef GALX_LDS_PAD
-#define GALX_LDS_PAD 0
-#endif
-#ifndef GALX_MATS_T
-#define GALX_MATS_T 1
-#endif
-#ifndef GALX_LDS_SWZ
-#define GALX_LDS_SWZ 0
-#endif
-// k_rhs: the node metrics issued with the state, ahead of the lifting (0 off, 1 mixed form and, with GALX_GEO_REG,
-// the FP64 split form, 2 every form)
-#ifndef GALX_PREFETCH_GEO
-#define GALX_PREFETCH_GEO 1
-#endif
-// k_lift: gradients of the surface nodes only (0 off, 1 FP64 form only, 2 both forms)
-#ifndef GALX_LIFT_SURF
-#define GALX_LIFT_SURF 2
-#endif
-// k_lift, surface-node gradients: the metrics read from the surface-node copy of the geometry (1 on)
-#ifndef GALX_SURF_GEO
-#define GALX_SURF_GEO 1
-#endif
-// k_rhs, FP64 split form: the metrics and sJ loaded with the state stay in registers for the volume terms and the
-// update instead of being read again (1 on; measured on MI355X at N=7: 51 B/DOF fewer HBM reads, 7 % faster)
-#ifndef GALX_GEO_REG
-#define GALX_GEO_REG 1
-#endif
-// k_rhs, FP64: the own trace records of the face nodes from the node threads through LDS (1 on).  Measured on
-// MI355X at N=7: 1.66-1.68 instead of 1.57-1.58 ms, as keeping the state live through the volume terms spills
-// 20 bytes per lane; off by default.
-#ifndef GALX_OWN_LDS
-#define GALX_OWN_LDS 0
-#endif
-// k_rhs, split form: each pair of an i- or j-line formed once and shared between the two lanes (1 on)
-#ifndef GALX_PAIR_SPLIT
-#define GALX_PAIR_SPLIT 1
-#endif
-// k_rhs, mixed split form: the nine metric terms of every node stay in LDS (FP32) from the start, so the volume
-// terms read them there instead of from memory a second time (1 on)
-#ifndef GALX_GEO_LDS
-#define GALX_GEO_LDS 1
-#endif
 
 namespace galx {
 
@@ -85,28 +43,25 @@ struct Dm {
 __host__ __device__ constexpr int surface_count(int N) { return 2 * (N + 1) * (N + 1) + (N - 1) * 4 * N; }
 
 // LDS layout of the per-node arrays read along element lines (lifting, split and viscous volume terms) and of
-// the D-matrices: node (i,j,k) at i + np*(j + n*k) with row stride np = n + GALX_LDS_PAD.  A padded stride (n+1)
-// spreads the rows of an i-line read over more LDS banks, but measured slower at N=7 (k_rhs FP64 1.99 vs 1.84 ms,
-// mixed 1.31 vs 1.26 ms), so the default is the dense layout (GALX_LDS_PAD 0).
-// With GALX_LDS_SWZ (n a power of two) the i index is stored XOR-swizzled by j, (i ^ j) + np*(j + n*k): the
-// lanes of an i-line read (same l, different j) then hit different banks without padding.  Measured at N=7:
-// k_rhs 1.86 vs 1.82 ms in FP64 and 1.29 vs 1.26 ms mixed, so it is off by default like the padding.
+// the D-matrices: node (i,j,k) at i + n*(j + n*k), the order of the element planes, row stride np = n.  Measured
+// at N=7 and not used: a padded row stride n+1 (k_rhs FP64 1.99 vs 1.84 ms, mixed 1.31 vs 1.26 ms) and an XOR
+// swizzle of i by j (FP64 1.86 vs 1.82 ms, mixed 1.29 vs 1.26 ms), both meant to spread an i-line read over more
+// LDS banks.
 template <int N>
 struct Lay {
-    static constexpr int n = N + 1, np = n + GALX_LDS_PAD, n3 = np * n * n;  // n3: padded array length
-    static constexpr bool swz = GALX_LDS_SWZ != 0 && (n & (n - 1)) == 0;
-    __device__ static __forceinline__ int idx(int i, int j, int k) { return (swz ? (i ^ j) : i) + np * (j + n * k); }
-    __device__ static __forceinline__ int at(int node) { return idx(node % n, (node / n) % n, node / (n * n)); }
-    // node l of the direction-d line through (i,j,k): C + (swizzled l) + s*l, with C, x, a, s fixed per line
+    static constexpr int n = N + 1, np = n, n3 = n * n * n;
+    __device__ static __forceinline__ int idx(int i, int j, int k) { return i + n * (j + n * k); }
+    __device__ static __forceinline__ int at(int node) { return node; }
+    // node l of the direction-d line through (i,j,k): C + s*l
     struct Line {
-        int C, x, a, s;
-        __device__ __forceinline__ int operator()(int l) const { return C + (a ? (l ^ x) : 0) + s * l; }
+        int C, s;
+        __device__ __forceinline__ int operator()(int l) const { return C + s * l; }
     };
     __device__ static __forceinline__ Line line(int d, int i, int j, int k)
     {
-        if (d == 0) return swz ? Line{np * (j + n * k), j, 1, 0} : Line{np * (j + n * k), 0, 0, 1};
-        if (d == 1) return swz ? Line{np * n * k, i, 1, np} : Line{i + np * n * k, 0, 0, np};
-        return Line{(swz ? (i ^ j) : i) + np * j, 0, 0, np * n};
+        if (d == 0) return Line{n * (j + n * k), 1};
+        if (d == 1) return Line{i + n * n * k, n};
+        return Line{i + n * j, n * n};
     }
 };
 
@@ -170,7 +125,7 @@ struct Stage {
 // consecutive entries instead of entries n apart, and k_rhs/k_lift run 2.6 %/1.5 % faster in FP64.  The adjoint
 // reads (idx, m) in its contractions and keeps the column-major storage (TR false).  Leading dimension
 // Lay<N>::np.
-template <int N, typename T, bool TR = (GALX_MATS_T != 0)>
+template <int N, typename T, bool TR = true>
 struct Mats {
     static constexpr int ld = Lay<N>::np;
     T DT[ld * (N + 1)];
@@ -183,7 +138,7 @@ struct Mats {
 };
 
 // D_T alone, for a kernel that only differentiates (k_lift): 1 KB less LDS at N=7 in FP64
-template <int N, typename T, bool TR = (GALX_MATS_T != 0)>
+template <int N, typename T, bool TR = true>
 struct MatsD {
     static constexpr int ld = Lay<N>::np;
     T DT[ld * (N + 1)];
@@ -504,21 +459,15 @@ __global__ void __launch_bounds__(Dm<N>::block, (LiftWaves<N, T>::value))
     load_mats<N, T>(M, mats, tid, D::block);
     const Gas<T> gas = make_gas<T>(gd);
     T uc[5], pr[6], H;
-    // only the surface nodes' gradients are used (for F_v.n on the faces): with `surf` the first Surf<N>::count
-    // threads evaluate them, one surface node each, and the interior nodes' gradients and metrics are skipped
-    // (measured on MI355X at N=7 with the metrics from the surface-node copy: 8 % faster in FP64, 12 % in the
-    // mixed form; with the metrics from the volume planes, whose 64-byte rows all hold surface nodes, 1.8 % faster
-    // in FP64 and 3 % slower in the mixed form)
-    constexpr bool surf = GALX_LIFT_SURF == 2 || (GALX_LIFT_SURF == 1 && sizeof(T) == 8);
-    const bool gnode = surf ? tid < Surf<N>::count : tid < n3;
-    const int gn = surf ? (gnode ? surface_node<N>(tid) : 0) : tid;
+    // only the surface nodes' gradients are used (for F_v.n on the faces): the first Surf<N>::count threads
+    // evaluate them, one surface node each, with the metrics from the surface-node copy of the geometry, and the
+    // interior nodes' gradients and metrics are skipped (measured on MI355X at N=7 against every node's gradient:
+    // 8 % faster in FP64, 12 % in the mixed form; with the metrics from the volume planes, whose 64-byte rows all
+    // hold surface nodes, 1.8 % faster in FP64 and 3 % slower in the mixed form)
+    const bool gnode = tid < Surf<N>::count;
+    const int gn = gnode ? surface_node<N>(tid) : 0;
     T gm[10];
-    if (gnode) {
-        if (surf && GALX_SURF_GEO)
-            load_surf_geo<N, T>(e, tid, mesh, gm);
-        else
-            load_node_geo<N, T>(e, gn, mesh, gm);
-    }
+    if (gnode) load_surf_geo<N, T>(e, tid, mesh, gm);
     if (tid < n3) {
         load_state<N, T>(e, tid, U, gd, uc, pr, H);
         const int pt = Lay<N>::at(tid);
@@ -662,17 +611,14 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
     constexpr int np = Lay<N>::np, n3p = Lay<N>::n3;  // padded node layout of the line-read arrays
     constexpr int w_a = 4 * n3p + 6 * 7 * n2;         // lifting
-    // mixed split form: the metrics of all three directions in s_geo for the whole kernel (needs the prefetch)
-    constexpr bool geo_lds = GALX_GEO_LDS && SPLIT && sizeof(T) == 4 &&
-                             (GALX_PREFETCH_GEO == 2 || GALX_PREFETCH_GEO == 1);
+    // mixed split form: the nine metric terms of every node stay in LDS (FP32, halves) in s_geo for the whole
+    // kernel, so the volume terms read them there instead of from memory a second time (7.8 % faster at N=7)
+    constexpr bool geo_lds = SPLIT && sizeof(T) == 4;
     constexpr int w_c = SPLIT ? (geo_lds ? 10 : 13) * n3p : 5 * n3p;  // volume fluxes / split-form partner data
-    // FP64: the own trace records of the face nodes reach the face tasks through LDS, formed by the node threads
-    // from the state they hold (uc is the FP64 state itself there), instead of a second gather of U
-    constexpr bool own_lds = GALX_OWN_LDS && sizeof(T) == 8;
-    constexpr int w_d = (own_lds ? 2 : 1) * 6 * 5 * n2;  // face results (and own records), new traces
+    constexpr int w_d = 6 * 5 * n2;  // face results, new traces
     // split form: the i- and j-lines lie inside one 64-lane wave (n and n^2 divide 64: N = 1, 3, 7), so the pairs of
     // those lines are shared between lanes instead of formed twice
-    constexpr bool pair_lines = GALX_PAIR_SPLIT && SPLIT && 64 % n == 0 && 64 % n2 == 0;
+    constexpr bool pair_lines = SPLIT && 64 % n == 0 && 64 % n2 == 0;
     constexpr int w_ac = w_a > w_c ? w_a : w_c;
     constexpr int lds_w = w_ac > w_d ? w_ac : w_d;
     __shared__ T s_w[lds_w];
@@ -692,10 +638,11 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     const Gas<T> gas = make_gas<T>(gd);
     T uc[5], pr[6], Hme = T(0);
     // the node's metrics, issued with the state so that the lifting does not wait for them (measured on MI355X at
-    // N=7: 8 % faster in the mixed form; 0.8 % slower in FP64 as a prefetch alone, 7 % faster once the FP64 split
-    // form also keeps them for the volume terms and the update, GALX_GEO_REG)
-    constexpr bool pgeo =
-        GALX_PREFETCH_GEO == 2 || (GALX_PREFETCH_GEO == 1 && (sizeof(T) == 4 || (SPLIT && GALX_GEO_REG)));
+    // N=7: 8 % faster in the mixed form; 0.8 % slower in FP64 as a prefetch alone, so the FP64 weak form reads them
+    // where it needs them).  The FP64 split form keeps them in registers for the volume terms and the update
+    // (`geo_reg`) instead of reading them a second time: 51 B/DOF fewer HBM reads and 7 % faster at N=7.
+    constexpr bool pgeo = sizeof(T) == 4 || SPLIT;
+    constexpr bool geo_reg = SPLIT && sizeof(T) == 8;
     T gm[10];
     if (active) {
         if (pgeo) load_node_geo<N, T>(e, tid, mesh, gm);
@@ -778,10 +725,8 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                 } else {
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
-                        if (pgeo && GALX_GEO_REG)  // the prefetched values, kept in registers since the load
-                            mme[c] = d == 0 ? gm[c] : (d == 1 ? gm[3 + c] : gm[6 + c]);
-                        else
-                            mme[c] = T(ge[(3 * d + c) * n3]);
+                        // the prefetched values, kept in registers since the load (geo_reg)
+                        mme[c] = d == 0 ? gm[c] : (d == 1 ? gm[3 + c] : gm[6 + c]);
                         hm[c] = T(0.5) * mme[c];
                         s_m[c * n3p + pt] = hm[c];
                     }
@@ -863,22 +808,6 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 
     // ---- face fluxes (FillFlux) and surface integral (SurfInt_Device) ----
     T* s_res = s_w;  // [6][5][n2]
-    T* s_own = s_w + 6 * 5 * n2;  // [6][5][n2] own trace records (own_lds)
-    if (own_lds) {
-        // the volume terms ended with a barrier: s_w is free
-        if (active) {
-            double c[5] = {double(uc[0]), double(uc[1]), double(uc[2]), double(uc[3]), double(uc[4])}, tr[5];
-            trace_record<false>(c, gd.kappa, gd.pref, tr);
-#pragma unroll
-            for (int ls = 1; ls <= 6; ls++) {
-                int ab;
-                if (!on_face<N>(ls, i, j, k, ab)) continue;
-#pragma unroll
-                for (int v = 0; v < 5; v++) s_own[(ls - 1) * 5 * n2 + v * n2 + ab] = T(tr[v]);
-            }
-        }
-        __syncthreads();
-    }
     for (int t = tid; t < D::nf; t += D::block) {
         int ls = t / n2 + 1, ab = t - (ls - 1) * n2, a = ab % n, b = ab / n;
         int node = face_vol_node<N>(ls, a, b);
@@ -887,12 +816,11 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         int flip = c & 7;
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
         T Uo[5], fvo[4], Un[5], fvn[4];
-        if (own_lds) {
-#pragma unroll
-            for (int v = 0; v < 5; v++) Uo[v] = s_own[(ls - 1) * 5 * n2 + v * n2 + ab];
-        } else {
+        {
             // from U at the face node, not from the own slot of the trace buffer: the same record, but that read
-            // goes to memory (measured 1.5 % slower in FP64 on MI355X)
+            // goes to memory (measured 1.5 % slower in FP64 on MI355X).  Formed by the node threads from the state
+            // they hold and passed through LDS instead, it measured 6 % slower in FP64 at N=7: the state then
+            // stays live through the volume terms and spills.
             const double* uo = U + (long)e * 5 * n3 + node;
             double c[5], tr[5];
 #pragma unroll
@@ -932,7 +860,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         double* ut_out = (st.mode == 0 ? Ut : Kreg) + (long)e * 5 * n3 + tid;
         double* un = Uout + (long)e * 5 * n3 + tid;
         double kv[5] = {0, 0, 0, 0, 0}, uv[5] = {0, 0, 0, 0, 0};
-        const double sJd = (pgeo && GALX_GEO_REG && sizeof(T) == 8) ? double(gm[9]) : ge[9 * n3];
+        const double sJd = geo_reg ? double(gm[9]) : ge[9 * n3];
         if (st.mode != 0) {
             const double* uo = U + (long)e * 5 * n3 + tid;
 #pragma unroll

@@ -19,6 +19,8 @@ from ._lib import GalxConfig, check, lib
 FORMS = {"weak": _lib.GALX_FORM_WEAK, "split": _lib.GALX_FORM_SPLIT_PI}
 RIEMANNS = {"lf": _lib.GALX_RIEMANN_LF, "roe": _lib.GALX_RIEMANN_ROE, "roeentropyfix": _lib.GALX_RIEMANN_ROEEF}
 PRECISIONS = {"fp64": _lib.GALX_PREC_FP64, "mixed": _lib.GALX_PREC_MIXED}
+# the exact functions whose manufactured source CalcSource implements (exactfunc.f90:131-139)
+SOURCE_FUNCS = (4, 41, 42, 43)
 
 
 def _dptr(a):
@@ -69,7 +71,20 @@ class DGOperator:
 
     # InitMesh counterpart
     def set_mesh(self, mesh):
+        """Copy the mesh (reference layouts) to the device.  The mesh must be built for this operator's degree:
+        galx_set_mesh reads every array at the sizes the context's N implies."""
         m = mesh
+        if m.N != self.N:
+            raise ValueError(f"mesh is built for N={m.N}, the operator for N={self.N}")
+        n, nE, nS = self.n, int(m.nElems), int(m.nSides)
+        shapes = {"ElemToSide": (3, 6, nE), "SideToElem": (5, nS), "S2V2": (2, n, n, 5, 6),
+                  "Metrics_fTilde": (3, n, n, n, nE), "Metrics_gTilde": (3, n, n, n, nE),
+                  "Metrics_hTilde": (3, n, n, n, nE), "sJ": (n, n, n, nE), "NormVec": (3, n, n, nS),
+                  "TangVec1": (3, n, n, nS), "TangVec2": (3, n, n, nS), "SurfElem": (n, n, nS)}
+        for name, shape in shapes.items():
+            got = np.shape(getattr(m, name))
+            if tuple(got) != shape:
+                raise ValueError(f"mesh.{name} has shape {tuple(got)}, expected {shape}")
         keep = [_F(m.ElemToSide, np.int32), _F(m.SideToElem, np.int32), _F(m.S2V2, np.int32),
                 _F(m.Metrics_fTilde), _F(m.Metrics_gTilde), _F(m.Metrics_hTilde), _F(m.sJ), _F(m.NormVec),
                 _F(m.TangVec1), _F(m.TangVec2), _F(m.SurfElem)]
@@ -88,13 +103,18 @@ class DGOperator:
     # InitExactFunc's doCalcSource + CalcSource (src/equations/navierstokes/idealgas/exactfunc.f90:131-139,915-1111)
     def set_source(self, exactFunc, AdvVel=(0.0, 0.0, 0.0), Elem_xGP=None):
         """Manufactured source of ExactFunc 4, 41, 42 or 43 on every later RHS evaluation, at the stage time;
-        any other exactFunc switches it off.  Elem_xGP(3,0:N,0:N,0:N,nElems): the node coordinates."""
+        any other exactFunc switches it off.  Elem_xGP(3,0:N,0:N,0:N,nElems): the node coordinates, required
+        when the source is switched on."""
         adv = _F(np.asarray(AdvVel, dtype=np.float64).reshape(3))
+        on = int(exactFunc) in SOURCE_FUNCS
         if Elem_xGP is None:
-            x = np.zeros(1, order="F")
+            if on and self.nElems is not None:
+                raise ValueError(f"ExactFunc {exactFunc} needs the node coordinates Elem_xGP")
+            x = np.zeros(1, order="F")  # not read: no source, or no mesh yet (the library refuses the call)
         else:
             x = _F(Elem_xGP)
-            assert x.shape == (3, self.n, self.n, self.n, self.nElems), x.shape
+            if self.nElems is not None and x.shape != (3, self.n, self.n, self.n, self.nElems):
+                raise ValueError(f"Elem_xGP has shape {x.shape}, expected {(3, self.n, self.n, self.n, self.nElems)}")
         check(lib().galx_set_source(self._h, int(exactFunc), _dptr(adv), _dptr(x)), "galx_set_source")
 
     def get_state(self):

@@ -71,7 +71,8 @@ int galx_create(const galx_config *cfg, galx_ctx **out);
 int galx_destroy(galx_ctx *ctx);
 
 /* InitMesh counterpart (src/mesh/mesh.f90:307-451): copy connectivity and geometry (host pointers, reference
- * layouts) into the device layout.  Validates SideToElem/ElemToSide consistency. */
+ * layouts, sized for the context's N) into the device layout.  Validates SideToElem/ElemToSide consistency.  A
+ * failed call leaves nothing allocated and may be repeated; a successful one cannot. */
 int galx_set_mesh(galx_ctx *ctx, int nElems, int nSides, const int *ElemToSide, const int *SideToElem,
                   const int *S2V2, const double *Metrics_fTilde, const double *Metrics_gTilde,
                   const double *Metrics_hTilde, const double *sJ, const double *NormVec, const double *TangVec1,
@@ -80,7 +81,9 @@ int galx_set_mesh(galx_ctx *ctx, int nElems, int nSides, const int *ElemToSide, 
 /* CalcSource (src/equations/navierstokes/idealgas/exactfunc.f90:915-1111) as switched on by InitExactFunc
  * (exactfunc.f90:131-139): for exactFunc 4, 41, 42, 43 every later RHS evaluation adds the manufactured source of
  * that exact function at its stage time (timestep.f90:87-90); any other value switches it off.  AdvVel(3) and
- * Elem_xGP(3,0:N,0:N,0:N,nElems) (the node coordinates, reference layout) are host arrays; needs the mesh. */
+ * Elem_xGP(3,0:N,0:N,0:N,nElems) (the node coordinates, reference layout) are host arrays; needs the mesh.  When
+ * the source is switched on, Elem_xGP must hold 3*nElems*(N+1)^3 doubles; switching off reads neither array.
+ * Discards a trajectory recorded by galx_lserk_forward_ckpt. */
 int galx_set_source(galx_ctx *ctx, int exactFunc, const double *AdvVel, const double *Elem_xGP);
 
 /* d_U = U and U = d_U (src/dg/dg.f90:166-169, src/timedisc/timedisc_func.f90:348-349), host arrays. */
@@ -143,7 +146,9 @@ int galx_rhs_vjp_device(galx_ctx *ctx, const double *dLam, double *dUbar);
 int galx_lserk_forward_ckpt(galx_ctx *ctx, double t, double dt, int nSteps, int ckptEvery);
 /* galx_lserk_adjoint: reverse sweep over the recorded trajectory (segments recomputed from their
  * checkpoints).  Ubar holds dJ/dU(final) on entry and dJ/dU(initial) on exit (U layout, host / device).
- * The context is left at the final state of the trajectory, which stays recorded. */
+ * The context is left at the final state of the trajectory, which stays recorded.  galx_set_state[_device],
+ * galx_lserk_step[s], galx_lserk_stage and galx_set_source discard the recording: the sweep then returns
+ * GALX_E_STATE. */
 int galx_lserk_adjoint(galx_ctx *ctx, double *Ubar);
 int galx_lserk_adjoint_device(galx_ctx *ctx, double *dUbar);
 /* galx_step_vjp: the adjoint of one TimeStepByLSERKW2 step (src/timedisc/timestep.f90:49-120) of size dt at the

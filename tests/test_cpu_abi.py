@@ -124,3 +124,38 @@ def test_fortran_host_binds_the_library():
     assert L.galx_version().decode() in res.stdout
     assert f"fp64 mask {L.galx_supported_degrees(0)}" in res.stdout
     assert f"mixed mask {L.galx_supported_degrees(1)}" in res.stdout
+
+
+def _detached_operator(N, nElems=None):
+    """A DGOperator with no library context: the host-side checks below run before any library call."""
+    from galaexi_amd import dg
+
+    op = dg.DGOperator.__new__(dg.DGOperator)
+    op._h = None
+    op.N, op.n = N, N + 1
+    op.nElems = op.nSides = nElems
+    return op
+
+
+def test_set_source_needs_coordinates_once_a_mesh_is_set():
+    """ExactFunc 4/41/42/43 read 3*nDOF node coordinates: without them (or with the wrong shape) the host mirror
+    raises instead of handing the library a short array."""
+    import numpy as np
+
+    op = _detached_operator(3, nElems=8)
+    for case in (4, 41, 42, 43):
+        with pytest.raises(ValueError, match="Elem_xGP"):
+            op.set_source(case, (0.3, 0.3, 0.3))
+        with pytest.raises(ValueError, match="shape"):
+            op.set_source(case, (0.3, 0.3, 0.3), np.zeros((3, 4, 4, 4, 7), order="F"))
+
+
+def test_set_mesh_checks_degree_and_shapes():
+    from galaexi_amd import mesh as M
+
+    m = M.box_mesh((2, 2, 2), 3)
+    with pytest.raises(ValueError, match="N=3"):
+        _detached_operator(4).set_mesh(m)
+    m.sJ = m.sJ[:, :, :, :7]
+    with pytest.raises(ValueError, match="sJ"):
+        _detached_operator(3).set_mesh(m)

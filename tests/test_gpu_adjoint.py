@@ -107,7 +107,8 @@ def test_vjp_annihilates_conservation_weights():
     op.close()
 
 
-@pytest.mark.parametrize("N,nel,mu0", [(3, (3, 2, 2), 2e-2), (4, (2, 2, 3), 0.0), (5, (2, 2, 2), 5e-3)])
+@pytest.mark.parametrize("N,nel,mu0", [(3, (3, 2, 2), 2e-2), (4, (2, 2, 3), 0.0), (5, (2, 2, 2), 5e-3),
+                                       (7, (2, 2, 2), 2e-2)])
 def test_vjp_vs_central_fd_of_oracle(N, nel, mu0):
     """lam . (R(U+eps v) - R(U-eps v))/(2 eps) of the FP64 oracle against Ubar . v: best over eps <= 1e-6."""
     say(f"case: vjp vs oracle FD, {nel} N={N} mu0={mu0}")
@@ -425,3 +426,112 @@ def test_step_vjp(precision):
         assert best <= 1e-6
     op.close()
     ref.close()
+
+
+def test_adjoint_discarded_by_state_changes():
+    """A recorded trajectory belongs to the state it ends at: set_state, a step, a stage or set_source afterwards
+    discard it, and the sweep then reports GALX_E_STATE instead of sweeping a stale trajectory."""
+    from galaexi_amd._lib import GalxError
+
+    say("case: stale trajectory refused, (2,2,2) N=3")
+    m, U = noisy_case((2, 2, 2), 3, seed=3)
+    op = gpu_op(m)
+    op.set_state(U)
+    dt, _ = op.CalcTimeStep()
+    for change in (lambda: op.set_state(U), lambda: op.TimeStepByLSERKW2(0.0, dt, 1),
+                   lambda: op.lserk_stage(0.0, dt, 1), lambda: op.set_source(0)):
+        op.set_state(U)
+        op.forward_ckpt(0.0, dt, 3, 2)
+        g = op.lserk_adjoint(U)  # the recording is usable, and stays so after a sweep
+        assert np.array_equal(op.lserk_adjoint(U), g)
+        change()
+        with pytest.raises(GalxError, match="trajectory"):
+            op.lserk_adjoint(U)
+    op.close()
+
+
+# C5 workload, reduced: curved Taylor-Green vortex (Re 1600, Ma 0.1) on [0,2pi]^3 with the 0.05 sine mapping, N=7,
+# 4^3 elements, 30 LSERK steps, checkpoints every 5; J = int 1/2 rho |u|^2 dV at the end.
+C5_GAS = dict(kappa=1.4, R=71.42857, Pr=0.72, mu0=1.0 / 1600.0, CFLScale=0.5, DFLScale=0.5)
+
+
+def c5_case(ne=4):
+    lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
+    m = M.box_mesh((ne, ne, ne), 7, lo=lo, hi=hi, deform_amp=0.05, deform_phase=(0.31, 0.17, 0.83))
+    return m, T.taylor_green(m.Elem_xGP, Ma=0.1, noise=1e-3, seed=1600)
+
+
+def c5_op(m, precision):
+    op = dgmod().DGOperator(7, form="split", riemann="lf", precision=precision, **C5_GAS)
+    op.set_mesh(m)
+    return op
+
+
+def well_conditioned_direction(g, seed, scale=2.0):
+    """scale * (g/|g| + 0.3 r/|r|), r seeded N(0,1): dJ.v is about scale*|g|, a sizeable part of J, so the central
+    difference is not lost to cancellation in J (a random direction alone gives dJ.v ~ 1e-5 J)."""
+    rng = np.random.default_rng(seed)
+    r = rng.standard_normal(g.shape)
+    return np.asfortranarray(scale * (g / np.linalg.norm(g.ravel()) + 0.3 * r / np.linalg.norm(r.ravel())))
+
+
+def consecutive_pass(errs, bar):
+    return any(errs[i] <= bar and errs[i + 1] <= bar for i in range(len(errs) - 1))
+
+
+def test_c5_reduced_n7_adjoint_vs_fd():
+    """The N=7 adjoint kernels on the C5 workload at 4^3: the FP64 checkpointed adjoint against central
+    differences of the FP64 time integration along a well-conditioned direction (dJ.v >= 1e-2 J), relative error
+    <= 1e-6 at two consecutive step sizes; then the C5 setting itself, the mixed-precision forward with the FP64
+    adjoint accumulated along it: bitwise independent of the checkpoint spacing, and within 1e-5 of the FP64
+    gradient and of the FP64 finite difference (the mixed trajectory's own deviation, which bounds how far the
+    two gradients can agree)."""
+    say("case: C5 reduced, curved TGV 4^3 N=7, 30 steps, ckpt 5, fp64 and mixed forward")
+    m, U0 = c5_case()
+    W = mass_weights(m)
+    steps, every = 30, 5
+    op = c5_op(m, "fp64")
+    op.set_state(U0)
+    dt, err = op.CalcTimeStep()
+    assert err == 0
+    op.forward_ckpt(0.0, dt, steps, every)
+    UN = op.get_state()
+    J0 = kinetic_energy(m, UN)
+    g = op.lserk_adjoint(kinetic_energy_grad(m, UN))
+    v = well_conditioned_direction(g, 5)
+    exact = float(np.sum(g * v))
+    say(f"  J={J0:.10e} dJ.v={exact:.6e} (dJ.v/J = {exact / J0:.2e})")
+    assert abs(exact) >= 1e-2 * J0
+
+    def J(X):
+        op.set_state(np.asfortranarray(X))
+        op.TimeStepByLSERKW2(0.0, dt, steps)
+        return kinetic_energy(m, op.get_state())
+
+    eps_list = [1e-1, 3e-2, 1e-2, 3e-3, 1e-3, 3e-4]
+    fd = {}
+    for eps in eps_list:
+        fd[eps] = (J(U0 + eps * v) - J(U0 - eps * v)) / (2 * eps)
+    errs = [abs(fd[e] - exact) / abs(exact) for e in eps_list]
+    say(f"  fp64 adjoint vs central FD, eps {eps_list}: rel errs {['%.1e' % e for e in errs]}")
+    assert consecutive_pass(errs, 1e-6), errs
+    fd_ref = fd[eps_list[int(np.argmin(errs))]]
+    op.close()
+    # the C5 setting: mixed forward, FP64 adjoint
+    mx = c5_op(m, "mixed")
+    res = []
+    for ck in (every, 1, steps):
+        mx.set_state(U0)
+        mx.forward_ckpt(0.0, dt, steps, ck)
+        UNm = mx.get_state()
+        res.append(mx.lserk_adjoint(kinetic_energy_grad(m, UNm)))
+    for r in res[1:]:
+        assert np.array_equal(r, res[0])
+    gm = res[0]
+    dev_state = np.linalg.norm((UNm - UN).ravel()) / np.linalg.norm(UN.ravel())
+    rel_g = np.linalg.norm((gm - g).ravel()) / np.linalg.norm(g.ravel())
+    rel_dir = abs(float(np.sum(gm * v)) - fd_ref) / abs(fd_ref)
+    say(f"  mixed forward: final state vs fp64 {dev_state:.1e}; gradient vs fp64 adjoint {rel_g:.1e}; "
+        f"dJ.v vs fp64 FD {rel_dir:.1e}")
+    assert rel_g <= 1e-5 and rel_dir <= 1e-5
+    mx.close()
