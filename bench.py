@@ -34,21 +34,25 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_rhs_k_lift_tgv_n7_48cubed_traff
 TRAFFIC_NE = 48
 
 
-def alg_bytes(N, b_t, face_geo=4):
+def alg_bytes(N, b_t, b_g=8, face_geo=4):
     """Algorithmic HBM bytes per DOF for one stage's two kernels (b_t: bytes per trace value, 8 in FP64, 4 in the
-    mixed form; phi = 6 n^2 / n^3 face nodes per volume node; face_geo: face geometry values k_rhs reads per face
-    node, 4 (normal, SurfElem) for the split form with Lax-Friedrichs, which is formed in Cartesian components,
-    10 (normal, both tangents, SurfElem) for the solvers that rotate into the normal frame):
-      k_lift: read U (5x8), metrics+sJ at the surface nodes (sigma 10x8, sigma = (n^3 - (n-2)^3) / n^3), face
-              normal+SurfElem (phi 4x8), neighbour U traces (phi 5 b_t); write own viscous normal-flux traces
+    mixed form; b_g: bytes per metric and face-geometry value the kernels compute with, 8 in FP64, 4 in the mixed
+    form, which reads the FP32 roundings of the FP64 geometry; phi = 6 n^2 / n^3 face nodes per volume node;
+    face_geo: face geometry values k_rhs reads per face node, 4 (normal, SurfElem) for the split form with
+    Lax-Friedrichs, which is formed in Cartesian components, 10 (normal, both tangents, SurfElem) for the solvers
+    that rotate into the normal frame):
+      k_lift: read U (5x8), metrics+sJ at the surface nodes (sigma 10 b_g, sigma = (n^3 - (n-2)^3) / n^3), face
+              normal+SurfElem (phi 4 b_g), neighbour U traces (phi 5 b_t); write own viscous normal-flux traces
               (phi 4 b_t)
-      k_rhs : read U (5x8), metrics+sJ (10x8), face geometry (phi face_geo x8), neighbour U and viscous-flux traces
-              (phi 9 b_t), LSERK register (5x8); write U (5x8), register (5x8), own next U traces (phi 5 b_t)"""
+      k_rhs : read U (5x8), metrics+sJ (10 b_g; the mixed form also reads the FP64 sJ for the update, +8), face
+              geometry (phi face_geo b_g), neighbour U and viscous-flux traces (phi 9 b_t), LSERK register (5x8);
+              write U (5x8), register (5x8), own next U traces (phi 5 b_t)"""
     n = N + 1
     phi = 6.0 / n
     sigma = (n ** 3 - max(n - 2, 0) ** 3) / n ** 3
-    lift = 5 * 8 + sigma * 10 * 8 + phi * (4 * 8 + 5 * b_t + 4 * b_t)
-    rhs = 5 * 8 + 10 * 8 + phi * face_geo * 8 + phi * 9 * b_t + 5 * 8 + 5 * 8 + 5 * 8 + phi * 5 * b_t
+    lift = 5 * 8 + sigma * 10 * b_g + phi * (4 * b_g + 5 * b_t + 4 * b_t)
+    rhs = (5 * 8 + 10 * b_g + (8 if b_g < 8 else 0) + phi * face_geo * b_g + phi * 9 * b_t + 5 * 8 + 5 * 8 + 5 * 8
+           + phi * 5 * b_t)
     return lift, rhs
 
 
@@ -132,12 +136,13 @@ def kinetic_energy_grad(W, U):
     return g
 
 
-def _adjoint_leg(m, U0, W, dt, steps, every, precision):
+def _adjoint_leg(m, U0, W, dt, steps, every, precision, storage="stages"):
     """Forward wall, checkpointed forward wall, reverse-sweep wall, dJ/dU0 and kernel timers of one context."""
     from galaexi_amd import dg
 
     op = dg.DGOperator(m.N, form="split", riemann="lf", precision=precision, **TGV_GAS)
     op.set_mesh(m)
+    op.set_adjoint_storage(storage)
     op.set_state(U0)
     op.TimeStepByLSERKW2(0.0, dt, 2)  # warm-up
     op.set_state(U0)
@@ -167,7 +172,10 @@ def run_adjoint(ne, steps, every, fd_eps, mixed_leg=True):
     central finite difference along a well-conditioned direction (the normalised gradient plus a seeded
     perturbation, so that dJ.v is a sizeable part of J) checks the gradient at every step size.  Mixed leg: the
     forward (and the segment recompute) in the mixed form, the adjoint accumulated in FP64 along that trajectory;
-    its gradient is compared with the FP64 leg's and with the FP64 finite differences."""
+    its gradient is compared with the FP64 leg's and with the FP64 finite differences.  Both legs keep each
+    segment's stage inputs (GALX_ADJ_STAGES); a third leg repeats the mixed one keeping only each segment's step
+    inputs and recomputing the stages of each step (GALX_ADJ_STEPS): less memory, more recompute, the same
+    gradient bit for bit."""
     N = 7
     m, U0 = tgv_case(ne, N, curved=True)
     W = mass_weights(m)
@@ -228,6 +236,17 @@ def run_adjoint(ne, steps, every, fd_eps, mixed_leg=True):
                                                    for e in fd_eps},
             "adjoint_kernels_ms_avg": {k: (v_[0] / v_[1] if v_[1] else 0.0) for k, v_ in kerm.items()},
             "device_memory_GB": rm["mem"] / 1e9,
+        }
+        op, rs = _adjoint_leg(m, U0, W, dt, steps, every, "mixed", storage="steps")
+        op.close()
+        out["mixed_forward_step_storage"] = {
+            "what": "the mixed leg with GALX_ADJ_STEPS: each segment's step inputs kept, each step's stages and the "
+                    "FP64 trace records recomputed before its VJPs",
+            "forward_s": rs["t_fwd"], "forward_with_checkpoints_s": rs["t_rec"], "adjoint_sweep_s": rs["t_adj"],
+            "gradient_over_forward": (rs["t_rec"] + rs["t_adj"]) / rs["t_fwd"],
+            "sweep_over_forward": rs["t_adj"] / rs["t_fwd"],
+            "gradient_bitwise_equal_to_stage_storage": bool(np.array_equal(rs["g"], gm)),
+            "device_memory_GB": rs["mem"] / 1e9,
         }
     return out
 
@@ -340,7 +359,7 @@ def main(argv=None):
         l2 = [float(np.linalg.norm((Ub[v] - Ua[v]).ravel()) / np.linalg.norm(Ua[v].ravel())) for v in range(5)]
         l2_all = float(np.linalg.norm((Ub - Ua).ravel()) / np.linalg.norm(Ua.ravel()))
         del Ua, Ub
-        mlift_b, mrhs_b = alg_bytes(N, 4)
+        mlift_b, mrhs_b = alg_bytes(N, 4, 4)
         mavg = {k: r32["kernels"][k][0] / r32["kernels"][k][1] / 1e3 for k in ("lift", "rhs")}
         out["compare"] = {"workload": f"TGV N={N} {a.ne}^3 ({nDOF} DOF), {a.compare_steps}+1 steps from t=0 with the "
                                       f"headline dt",

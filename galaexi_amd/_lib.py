@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libgalx.so")
 GALX_FORM_WEAK, GALX_FORM_SPLIT_PI = 0, 1
 GALX_RIEMANN_LF, GALX_RIEMANN_ROE, GALX_RIEMANN_ROEEF = 0, 1, 3
 GALX_PREC_FP64, GALX_PREC_MIXED = 0, 1
+GALX_ADJ_STAGES, GALX_ADJ_STEPS = 0, 1
 
 # every symbol declared in include/galx.h
 EXPORTS = (
@@ -22,7 +23,8 @@ EXPORTS = (
     "galx_surf_int", "galx_timers_enable", "galx_timer_read", "galx_timers_reset", "galx_memory_bytes",
     "galx_rhs_vjp", "galx_rhs_vjp_device", "galx_lserk_forward_ckpt", "galx_lserk_adjoint",
     "galx_lserk_adjoint_device", "galx_lserk_stage", "galx_get_gradients", "galx_get_gradients_device",
-    "galx_step_vjp", "galx_step_vjp_device", "galx_set_source",
+    "galx_step_vjp", "galx_step_vjp_device", "galx_set_source", "galx_set_adjoint_storage",
+    "galx_get_lifted_gradients", "galx_get_lifted_gradients_device",
 )
 
 
@@ -81,6 +83,7 @@ def lib():
         "galx_rhs_vjp": ([vp, dp, dp], i),
         "galx_rhs_vjp_device": ([vp, vp, vp], i),
         "galx_lserk_forward_ckpt": ([vp, d, d, i, i], i),
+        "galx_set_adjoint_storage": ([vp, i], i),
         "galx_lserk_adjoint": ([vp, dp], i),
         "galx_lserk_adjoint_device": ([vp, vp], i),
         "galx_step_vjp": ([vp, C.c_double, C.c_double, dp], i),
@@ -88,6 +91,8 @@ def lib():
         "galx_lserk_stage": ([vp, d, d, i], i),
         "galx_get_gradients": ([vp, dp, dp, dp], i),
         "galx_get_gradients_device": ([vp, vp, vp, vp], i),
+        "galx_get_lifted_gradients": ([vp, i, dp, dp, dp], i),
+        "galx_get_lifted_gradients_device": ([vp, i, vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

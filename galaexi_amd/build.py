@@ -13,8 +13,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgalx.so")
 
-DEFAULT_FP64 = (1, 2, 3, 4, 5, 6, 7, 8)
+# FP64: N <= 9 one volume node per thread, N = 10..13 the high-order kernels (the degrees of the reference's
+# p-convergence check, regressioncheck/checks/convtest/p_3D); the discrete adjoint for N <= 8
+DEFAULT_FP64 = tuple(range(1, 14))
 DEFAULT_MIXED = (3, 5, 7)
+DEFAULT_ADJ = tuple(range(1, 9))
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -35,11 +38,14 @@ def deps():
     return [os.path.join(CSRC, f) for f in os.listdir(CSRC)] + [os.path.join(HERE, "..", "include", "galx.h")]
 
 
-def build(degrees=DEFAULT_FP64, mixed_degrees=DEFAULT_MIXED, out=LIB, verbose=False, resource_report=None):
+def build(degrees=DEFAULT_FP64, mixed_degrees=DEFAULT_MIXED, out=LIB, verbose=False, resource_report=None,
+          adj_degrees=DEFAULT_ADJ):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     xs = " ".join(f"X({d})" for d in degrees)
     xm = " ".join(f"X({d})" for d in mixed_degrees)
-    cmd = [hipcc] + HIPCC_FLAGS + [f"-DGALX_DEGREES_FP64={xs}", f"-DGALX_DEGREES_MIXED={xm}"]
+    xa = " ".join(f"X({d})" for d in adj_degrees if d in degrees)
+    cmd = [hipcc] + HIPCC_FLAGS + [f"-DGALX_DEGREES_FP64={xs}", f"-DGALX_DEGREES_MIXED={xm}",
+                                   f"-DGALX_DEGREES_ADJ={xa}"]
     if resource_report:
         cmd.append("-Rpass-analysis=kernel-resource-usage")
     tmp = out + ".tmp"

@@ -194,6 +194,7 @@ __device__ __forceinline__ void lift_face_tasks(int e, int tid, const Mesh& mesh
                                                 double* s_f)
 {
     using D = Dm<N>;
+    static_assert(D::node_per_thread, "one volume node per thread: N <= 9");
     constexpr int n2 = D::n2;
     for (int t = tid; t < D::nf; t += D::block) {
         int ls = t / n2 + 1, ab = t - (ls - 1) * n2;
@@ -268,6 +269,7 @@ __global__ void __launch_bounds__(Dm<N>::block)
               double* __restrict__ mutr, const double* __restrict__ Uprev, const double* __restrict__ nghtr, GasD gd)
 {
     using D = Dm<N>;
+    static_assert(D::node_per_thread, "one volume node per thread: N <= 9");
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
     __shared__ double s_l[5 * n3];  // -sJ * lam
     __shared__ double s_f[6 * 4 * n2];
@@ -361,6 +363,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
                double* __restrict__ Ubar, int accumulate, int kscale, double a, double* __restrict__ nghtr)
 {
     using D = Dm<N>;
+    static_assert(D::node_per_thread, "one volume node per thread: N <= 9");
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
     constexpr int np = Lay<N>::np, n3p = Lay<N>::n3;  // matrices and the lifting input: padded layout
     __shared__ double s_w[AdjLds<N>::w];
@@ -645,6 +648,7 @@ __global__ void __launch_bounds__(Dm<N>::block)
                double* __restrict__ Ubar)
 {
     using D = Dm<N>;
+    static_assert(D::node_per_thread, "one volume node per thread: N <= 9");
     constexpr int n2 = D::n2, n3 = D::n3;
     __shared__ double s_f[6 * 4 * n2];
     const int e = blockIdx.x, tid = threadIdx.x;

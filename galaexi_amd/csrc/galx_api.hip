@@ -14,6 +14,7 @@
 #include "../../include/galx.h"
 #include "galx_adjoint.hpp"
 #include "galx_kernels.hpp"
+#include "galx_kernels_ho.hpp"
 
 using namespace galx;
 
@@ -143,20 +144,35 @@ __global__ void k_planes_to_ref(long nDOF, int n3, const double* __restrict__ sr
     for (int v = 0; v < 5; v++) dst[5 * x + v] = src[(e * 5 + v) * n3 + node];
 }
 
+// FP32 rounding (round to nearest even) of an FP64 array: the mixed form's geometry copies
+__global__ void k_round_f32(long n, const double* __restrict__ src, float* __restrict__ dst)
+{
+    long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n) dst[x] = float(src[x]);
+}
+
 // gradient planes [e][12][n3] -> gradUx/y/z(4,0:N,0:N,0:N,nElems) (lifting variables u,v,w,T)
-__global__ void k_grad_to_ref(long nDOF, int n3, const double* __restrict__ g, double* __restrict__ gx,
+// nv = 4: (u,v,w,T) (PP_OPTLIFT); nv = 5: (rho,u,v,w,T) (the default lifting, eos.h:133-142), the density
+// gradient from planes 12..14
+__global__ void k_grad_to_ref(long nDOF, int n3, int nv, const double* __restrict__ g, double* __restrict__ gx,
                               double* __restrict__ gy, double* __restrict__ gz)
 {
     long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= nDOF) return;
     long e = x / n3;
     int node = (int)(x - e * n3);
-    const double* s = g + e * 12 * n3 + node;
+    const double* s = g + e * 15 * n3 + node;
+    const int o = nv - 4;
+    if (o) {
+        gx[nv * x] = s[12 * n3];
+        gy[nv * x] = s[13 * n3];
+        gz[nv * x] = s[14 * n3];
+    }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        gx[4 * x + q] = s[q * n3];
-        gy[4 * x + q] = s[(4 + q) * n3];
-        gz[4 * x + q] = s[(8 + q) * n3];
+        gx[nv * x + o + q] = s[q * n3];
+        gy[nv * x + o + q] = s[(4 + q) * n3];
+        gz[nv * x + o + q] = s[(8 + q) * n3];
     }
 }
 
@@ -220,6 +236,7 @@ __global__ void __launch_bounds__(256) k_pressure_sum(long nDOF, int n3, double 
 struct KernelSet {
     bool ok = false;
     int block = 0;
+    size_t ws_per_elem = 0;  // bytes of workspace per element (the high-order kernels, N >= 10)
     void (*trace)(hipStream_t, int, const double*, Mesh, void*, GasD) = nullptr;
     void (*lift)(hipStream_t, int, const double*, Mesh, const void*, void*, const void*, GasD, double, double) = nullptr;
     void (*rhs)(hipStream_t, int, const double*, double*, Mesh, const void*, const void*, void*, double*, double*,
@@ -228,6 +245,8 @@ struct KernelSet {
                double*, int*) = nullptr;
     void (*grad)(hipStream_t, int, const double*, Mesh, const void*, const void*, GasD, double, double,
                  double*) = nullptr;
+    void (*grad_rho)(hipStream_t, int, const double*, Mesh, const void*, const double*, double, double,
+                     double*) = nullptr;
 };
 
 template <int N, typename T, bool SPLIT>
@@ -240,16 +259,26 @@ struct Launch {
     static void lift(hipStream_t s, int nE, const double* U, Mesh m, const void* utr, void* vtr, const void* mats,
                      GasD g, double LHm, double LHp)
     {
-        hipLaunchKernelGGL((k_lift<N, T, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr, (T*)vtr,
-                           (const T*)mats, g, T(LHm), T(LHp));
+        if constexpr (D::node_per_thread)
+            hipLaunchKernelGGL((k_lift<N, T, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr, (T*)vtr,
+                               (const T*)mats, g, T(LHm), T(LHp));
+        else
+            hipLaunchKernelGGL((k_lift_ho<N, T, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr, (T*)vtr,
+                               (const T*)mats, g, T(LHm), T(LHp));
     }
     template <int RIE>
     static void rhs_r(hipStream_t s, int nE, const double* U, double* Uout, Mesh m, const void* utr, const void* vtr,
                       void* utr_next, double* Ut, double* K, const void* mats, GasD g, double LHm, double LHp,
                       Stage st)
     {
-        hipLaunchKernelGGL((k_rhs<N, T, T, SPLIT, RIE>), dim3(nE), dim3(D::block), 0, s, U, Uout, m, (const T*)utr,
-                           (const T*)vtr, (T*)utr_next, Ut, K, (const T*)mats, g, T(LHm), T(LHp), st);
+        if constexpr (D::node_per_thread)
+            hipLaunchKernelGGL((k_rhs<N, T, T, SPLIT, RIE>), dim3(nE), dim3(D::block), 0, s, U, Uout, m,
+                               (const T*)utr, (const T*)vtr, (T*)utr_next, Ut, K, (const T*)mats, g, T(LHm), T(LHp),
+                               st);
+        else
+            hipLaunchKernelGGL((k_rhs_ho<N, T, T, SPLIT, RIE>), dim3(nE), dim3(D::block), 0, s, U, Uout, m,
+                               (const T*)utr, (const T*)vtr, (T*)utr_next, Ut, K, (const T*)mats, g, T(LHm), T(LHp),
+                               st);
     }
     static void rhs(hipStream_t s, int nE, const double* U, double* Uout, Mesh m, const void* utr, const void* vtr,
                     void* utr_next, double* Ut, double* K, const void* mats, GasD g, int riemann, double LHm,
@@ -271,15 +300,27 @@ struct Launch {
     static void grad(hipStream_t s, int nE, const double* U, Mesh m, const void* utr, const void* mats, GasD g,
                      double LHm, double LHp, double* gvol)
     {
-        hipLaunchKernelGGL((k_grad<N, T, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr, (const T*)mats, g,
-                           T(LHm), T(LHp), gvol);
+        if constexpr (D::node_per_thread)
+            hipLaunchKernelGGL((k_grad<N, T, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr,
+                               (const T*)mats, g, T(LHm), T(LHp), gvol);
+        else
+            hipLaunchKernelGGL((k_grad_ho<N, T, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr,
+                               (const T*)mats, g, T(LHm), T(LHp), gvol);
+    }
+    static void grad_rho(hipStream_t s, int nE, const double* U, Mesh m, const void* utr, const double* DT64,
+                         double LHm, double LHp, double* gvol)
+    {
+        hipLaunchKernelGGL((k_grad_rho<N, T>), dim3(nE), dim3(D::block), 0, s, U, m, (const T*)utr, DT64, LHm, LHp,
+                           gvol);
     }
     static KernelSet make()
     {
         KernelSet k;
         k.grad = &grad;
+        k.grad_rho = &grad_rho;
         k.ok = true;
         k.block = D::block;
+        k.ws_per_elem = D::node_per_thread ? 0 : HoWs<N>::size * sizeof(T);
         k.trace = &trace;
         k.lift = &lift;
         k.rhs = &rhs;
@@ -341,6 +382,10 @@ struct AdjLaunch {
 #ifndef GALX_DEGREES_MIXED
 #define GALX_DEGREES_MIXED X(3) X(5) X(7)
 #endif
+// degrees of the discrete adjoint (FP64 kernels; one volume node per thread)
+#ifndef GALX_DEGREES_ADJ
+#define GALX_DEGREES_ADJ X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8)
+#endif
 
 static KernelSet select_kernels(int N, int prec, int form)
 {
@@ -363,7 +408,7 @@ static AdjSet select_adjoint(int N)
 {
 #define X(NN) \
     if (N == NN) return AdjLaunch<NN>::make();
-    GALX_DEGREES_FP64
+    GALX_DEGREES_ADJ
 #undef X
     return AdjSet{};
 }
@@ -390,6 +435,8 @@ struct galx_ctx {
     // device buffers
     double *dU = nullptr, *dK = nullptr, *dUt = nullptr, *dScratch = nullptr;
     double *dGeo = nullptr, *dFgeo = nullptr, *dSgeo = nullptr;
+    float *dGeo32 = nullptr, *dFgeo32 = nullptr, *dSgeo32 = nullptr;  // mixed form: FP32 roundings of the three
+    void* dWs = nullptr;  // per-element workspace of the high-order kernels (N >= 10)
     int *dConn = nullptr, *dInv = nullptr;
     void *dUtr[2] = {nullptr, nullptr}, *dVtr = nullptr, *dMats = nullptr;
     double *dDtc = nullptr, *dDtv = nullptr, *dDtOut = nullptr;
@@ -402,6 +449,8 @@ struct galx_ctx {
     std::vector<double*> ckpt;   // U at steps 0, C, 2C, ... of the recorded trajectory
     std::vector<double*> stages; // stage inputs of one segment of C steps (5 per step)
     std::vector<double*> stageTr; // their FP64 trace records, kept from the recompute (FP64 form)
+    std::vector<double*> steps;   // GALX_ADJ_STEPS: the step inputs 1..C-1 of one segment
+    int adjStorage = GALX_ADJ_STAGES;
     double *dUfinal = nullptr;
     // mixed form: the adjoint kernels run in FP64 on FP64 trace records recomputed from each stage input,
     // with the FP64 matrices
@@ -474,7 +523,10 @@ static void flush_timers(galx_ctx* c)
     }
 }
 
-static Mesh mesh_of(galx_ctx* c) { return Mesh{c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dSgeo}; }
+static Mesh mesh_of(galx_ctx* c)
+{
+    return Mesh{c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dSgeo, c->dGeo32, c->dFgeo32, c->dSgeo32, c->dWs};
+}
 static GasD gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, c->pref}; }
 // the adjoint kernels and their FP64 trace records use the pressure itself (no offset)
 static GasD adj_gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, 0.0}; }
@@ -547,7 +599,12 @@ static void free_mesh(galx_ctx* c)
         dfree(*p);
         *p = nullptr;
     }
-    void** pv[] = {&c->dUtr[0], &c->dUtr[1], &c->dVtr};
+    float** pf[] = {&c->dGeo32, &c->dFgeo32, &c->dSgeo32};
+    for (float** p : pf) {
+        dfree(*p);
+        *p = nullptr;
+    }
+    void** pv[] = {&c->dUtr[0], &c->dUtr[1], &c->dVtr, &c->dWs};
     for (void** p : pv) {
         dfree(*p);
         *p = nullptr;
@@ -579,6 +636,12 @@ static int set_mesh_device(galx_ctx* c, int nElems, int nSides, const std::vecto
     if ((rc = dalloc(c, &c->dDtv, nElems * sizeof(double)))) return rc;
     if ((rc = dalloc(c, &c->dDtOut, 4 * sizeof(double)))) return rc;
     if ((rc = dalloc(c, &c->dErr, sizeof(int)))) return rc;
+    if (c->ks.ws_per_elem && (rc = dalloc(c, (char**)&c->dWs, c->ks.ws_per_elem * nElems))) return rc;
+    if (c->tsz == 4) {
+        if ((rc = dalloc(c, &c->dGeo32, 10 * nDOF * sizeof(float)))) return rc;
+        if ((rc = dalloc(c, &c->dFgeo32, 10 * nfn * sizeof(float)))) return rc;
+        if ((rc = dalloc(c, &c->dSgeo32, 10L * nElems * ns * sizeof(float)))) return rc;
+    }
     HIPCHK(hipMemcpy(c->dConn, conn.data(), conn.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->dInv, inv.data(), inv.size() * sizeof(int), hipMemcpyHostToDevice));
     // geometry into element / side planes, in chunks through two pinned staging buffers: the host fills one while
@@ -637,6 +700,15 @@ static int set_mesh_device(galx_ctx* c, int nElems, int nSides, const std::vecto
         HIPCHK(hipMemcpyAsync(c->dFgeo + s0 * 10 * n2, buf, nsc * 10 * n2 * sizeof(double), hipMemcpyHostToDevice,
                               c->stream));
         if ((rc = stg.issued(c->stream))) return rc;
+    }
+    if (c->tsz == 4) {
+        const std::pair<const double*, float*> cp[3] = {{c->dGeo, c->dGeo32}, {c->dFgeo, c->dFgeo32},
+                                                        {c->dSgeo, c->dSgeo32}};
+        const long len[3] = {10 * nDOF, 10 * nfn, 10L * nElems * ns};
+        for (int a = 0; a < 3; a++)
+            k_round_f32<<<dim3((unsigned)((len[a] + 255) / 256)), dim3(256), 0, c->stream>>>(len[a], cp[a].first,
+                                                                                           cp[a].second);
+        HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return GALX_OK;
@@ -714,7 +786,8 @@ int galx_create(const galx_config* cfg, galx_ctx** out)
         if ((rc = dalloc(c, (float**)&c->dMats, 3 * nn * sizeof(float)))) { galx_destroy(c); return rc; }
         e = hipMemcpy(c->dMats, mf.data(), 3 * nn * sizeof(float), hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess && c->as.ok) {
+    // FP64 matrices in every context: the adjoint kernels and the density gradient of the analysis use them
+    if (e == hipSuccess) {
         if ((rc = dalloc(c, &c->dMats64, 3 * nn * sizeof(double)))) { galx_destroy(c); return rc; }
         e = hipMemcpy(c->dMats64, md.data(), 3 * nn * sizeof(double), hipMemcpyHostToDevice);
     }
@@ -733,7 +806,8 @@ int galx_destroy(galx_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     flush_timers(c);
     void* ps[] = {c->dU,      c->dK,    c->dUt,   c->dScratch, c->dGeo,  c->dFgeo, c->dSgeo, c->dConn, c->dInv,
-                  c->dUtr[0], c->dUtr[1], c->dVtr, c->dMats,    c->dDtc, c->dDtv,  c->dDtOut, c->dErr};
+                  c->dUtr[0], c->dUtr[1], c->dVtr, c->dMats,    c->dDtc, c->dDtv,  c->dDtOut, c->dErr,
+                  c->dGeo32,  c->dFgeo32, c->dSgeo32, c->dWs};
     for (void* p : ps) dfree(p);
     void* pa[] = {c->dKbar,    c->dUbar,   c->dMutr,    c->dNghtr, c->dUfinal,
                   c->dGrad,    c->dGradRef, c->dMats64, c->dTr64,  c->dSrcX, c->dPsum};
@@ -741,6 +815,7 @@ int galx_destroy(galx_ctx* c)
     for (double* p : c->ckpt) dfree(p);
     for (double* p : c->stages) dfree(p);
     for (double* p : c->stageTr) dfree(p);
+    for (double* p : c->steps) dfree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return GALX_OK;
@@ -1101,45 +1176,62 @@ int galx_lserk_stage(galx_ctx* c, double t, double dt, int iStage)
     return GALX_OK;
 }
 
-static int grad_enqueue(galx_ctx* c, double* gx, double* gy, double* gz)
+static int grad_enqueue(galx_ctx* c, int nv, double* gx, double* gy, double* gz)
 {
     long nDOF = (long)c->nElems * c->n3;
     int rc;
-    if ((rc = dalloc(c, &c->dGrad, 12 * nDOF * sizeof(double)))) return rc;
+    if ((rc = dalloc(c, &c->dGrad, 15 * nDOF * sizeof(double)))) return rc;
     const Basis& B = c->basis;
     c->ks.grad(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[c->cur], c->dMats, gas_of(c), 1.0 / B.w[0],
                1.0 / B.w[c->N], c->dGrad);
-    k_grad_to_ref<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dGrad, gx, gy, gz);
+    if (nv == 5)
+        c->ks.grad_rho(c->stream, c->nElems, c->dU, mesh_of(c), c->dUtr[c->cur], c->dMats64, 1.0 / B.w[0],
+                       1.0 / B.w[c->N], c->dGrad);
+    k_grad_to_ref<<<dim3((unsigned)((nDOF + 255) / 256)), dim3(256), 0, c->stream>>>(nDOF, c->n3, nv, c->dGrad, gx,
+                                                                                     gy, gz);
     HIPCHK(hipGetLastError());
     return GALX_OK;
 }
 
-int galx_get_gradients_device(galx_ctx* c, double* dGx, double* dGy, double* dGz)
+int galx_get_lifted_gradients_device(galx_ctx* c, int nVarLifting, double* dGx, double* dGy, double* dGz)
 {
     if (!c || !dGx || !dGy || !dGz) return fail(GALX_E_ARG, "null argument");
+    if (nVarLifting != 4 && nVarLifting != 5) return fail(GALX_E_ARG, "nVarLifting must be 4 or 5");
     if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
     HIPCHK(hipSetDevice(c->cfg.device));
-    int rc = grad_enqueue(c, dGx, dGy, dGz);
+    int rc = grad_enqueue(c, nVarLifting, dGx, dGy, dGz);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     return GALX_OK;
 }
 
-int galx_get_gradients(galx_ctx* c, double* gx, double* gy, double* gz)
+int galx_get_lifted_gradients(galx_ctx* c, int nVarLifting, double* gx, double* gy, double* gz)
 {
     if (!c || !gx || !gy || !gz) return fail(GALX_E_ARG, "null argument");
+    if (nVarLifting != 4 && nVarLifting != 5) return fail(GALX_E_ARG, "nVarLifting must be 4 or 5");
     if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
     HIPCHK(hipSetDevice(c->cfg.device));
     long nDOF = (long)c->nElems * c->n3;
+    const size_t b = (size_t)nVarLifting * nDOF * sizeof(double);
     int rc;
-    if ((rc = dalloc(c, &c->dGradRef, 12 * nDOF * sizeof(double)))) return rc;
-    double *dx = c->dGradRef, *dy = dx + 4 * nDOF, *dz = dy + 4 * nDOF;
-    if ((rc = grad_enqueue(c, dx, dy, dz))) return rc;
-    HIPCHK(hipMemcpyAsync(gx, dx, 4 * nDOF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(gy, dy, 4 * nDOF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(gz, dz, 4 * nDOF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = dalloc(c, &c->dGradRef, 15 * nDOF * sizeof(double)))) return rc;
+    double *dx = c->dGradRef, *dy = dx + nVarLifting * nDOF, *dz = dy + nVarLifting * nDOF;
+    if ((rc = grad_enqueue(c, nVarLifting, dx, dy, dz))) return rc;
+    HIPCHK(hipMemcpyAsync(gx, dx, b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(gy, dy, b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(gz, dz, b, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return GALX_OK;
+}
+
+int galx_get_gradients_device(galx_ctx* c, double* dGx, double* dGy, double* dGz)
+{
+    return galx_get_lifted_gradients_device(c, 4, dGx, dGy, dGz);
+}
+
+int galx_get_gradients(galx_ctx* c, double* gx, double* gy, double* gz)
+{
+    return galx_get_lifted_gradients(c, 4, gx, gy, gz);
 }
 
 // CFLScaleAlpha / DFLScaleAlpha for Gauss-Lobatto nodes, Carpenter RK4-5 (src/timedisc/timedisc_vars.f90:105-109,
@@ -1213,7 +1305,8 @@ int galx_memory_bytes(galx_ctx* c, long long* bytes)
 static int adj_check(galx_ctx* c)
 {
     if (!c) return fail(GALX_E_ARG, "null context");
-    if (!c->as.ok) return fail(GALX_E_ARG, "the discrete adjoint is implemented for form SPLIT_PI, riemann LF");
+    if (!c->as.ok)
+        return fail(GALX_E_ARG, "the discrete adjoint is implemented for form SPLIT_PI, riemann LF, N <= 8");
     if (!c->stateSet) return fail(GALX_E_STATE, "set_state first");
     HIPCHK(hipSetDevice(c->cfg.device));
     long nDOF = (long)c->nElems * c->n3, nfn = (long)c->nSides * c->n2;
@@ -1222,18 +1315,23 @@ static int adj_check(galx_ctx* c)
     if ((rc = dalloc(c, &c->dUbar, 5 * nDOF * sizeof(double)))) return rc;
     if ((rc = dalloc(c, &c->dMutr, 2 * 5 * nfn * sizeof(double)))) return rc;
     if ((rc = dalloc(c, &c->dNghtr, 2 * 4 * nfn * sizeof(double)))) return rc;
-    if (c->tsz != 8 && (rc = dalloc(c, &c->dTr64, 2 * 5 * nfn * sizeof(double)))) return rc;
+    if ((c->tsz != 8 || c->adjStorage == GALX_ADJ_STEPS) &&
+        (rc = dalloc(c, &c->dTr64, 2 * 5 * nfn * sizeof(double))))
+        return rc;
     return GALX_OK;
 }
 
-// FP64 trace records of U for the adjoint kernels: the context's own in the FP64 form (valid for dU), recomputed
-// into dTr64 in the mixed form
+// FP64 trace records of U for the adjoint kernels: fp64_traces when the caller holds them (the FP64 form's own
+// records of its state, or the records kept by the recompute), else recomputed into dTr64
 static const double* adj_traces(galx_ctx* c, const double* U, const double* fp64_traces)
 {
-    if (c->tsz == 8) return fp64_traces;
+    if (fp64_traces) return fp64_traces;
     c->as.trace64(c->stream, c->nElems, U, mesh_of(c), c->dTr64, adj_gas_of(c));
     return c->dTr64;
 }
+
+// the context's own FP64 trace records of its current state, or null in the mixed form (FP32 records)
+static const double* own_fp64_traces(galx_ctx* c) { return c->tsz == 8 ? (const double*)c->dUtr[c->cur] : nullptr; }
 
 // Ubar (+)= (dR/dU(U))^T lam for U in element planes with its FP64 trace records utr; lam is consumed:
 // lam <- lam + bdt*Ubar first when Ubar accumulates (LSERK stage), and lam <- a*lam at the end when kscale.
@@ -1263,8 +1361,7 @@ int galx_rhs_vjp_device(galx_ctx* c, const double* dLam, double* dUbarOut)
     unsigned g = (unsigned)((nDOF + 255) / 256);
     k_ref_to_planes<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, dLam, c->dKbar);
     HIPCHK(hipGetLastError());
-    adj_apply(c, c->dU, adj_traces(c, c->dU, (const double*)c->dUtr[c->cur]), c->dKbar, c->dUbar, 0, 0.0, 0, 0.0,
-              nullptr, true);
+    adj_apply(c, c->dU, adj_traces(c, c->dU, own_fp64_traces(c)), c->dKbar, c->dUbar, 0, 0.0, 0, 0.0, nullptr, true);
     HIPCHK(hipGetLastError());
     k_planes_to_ref<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUbar, dUbarOut);
     HIPCHK(hipGetLastError());
@@ -1283,13 +1380,39 @@ int galx_rhs_vjp(galx_ctx* c, const double* lam, double* UbarOut)
     HIPCHK(hipMemcpyAsync(c->dScratch, lam, 5 * nDOF * sizeof(double), hipMemcpyHostToDevice, c->stream));
     k_ref_to_planes<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dScratch, c->dKbar);
     HIPCHK(hipGetLastError());
-    adj_apply(c, c->dU, adj_traces(c, c->dU, (const double*)c->dUtr[c->cur]), c->dKbar, c->dUbar, 0, 0.0, 0, 0.0,
-              nullptr, true);
+    adj_apply(c, c->dU, adj_traces(c, c->dU, own_fp64_traces(c)), c->dKbar, c->dUbar, 0, 0.0, 0, 0.0, nullptr, true);
     HIPCHK(hipGetLastError());
     k_planes_to_ref<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUbar, c->dScratch);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(UbarOut, c->dScratch, 5 * nDOF * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return GALX_OK;
+}
+
+// resize a list of equal device buffers to `count`, freeing the surplus
+static int resize_buffers(galx_ctx* c, std::vector<double*>& v, int count, size_t bytes, const char* what)
+{
+    int rc;
+    while ((int)v.size() > count) {
+        dfree(v.back());
+        c->bytes -= (long long)bytes;
+        v.pop_back();
+    }
+    while ((int)v.size() < count) {
+        double* p = nullptr;
+        if ((rc = dalloc(c, &p, bytes)))
+            return fail(rc, "%s (%d x %zu bytes; lower ckptEvery or nSteps): %s", what, count, bytes, g_err.c_str());
+        v.push_back(p);
+    }
+    return GALX_OK;
+}
+
+int galx_set_adjoint_storage(galx_ctx* c, int mode)
+{
+    if (!c) return fail(GALX_E_ARG, "null context");
+    if (mode != GALX_ADJ_STAGES && mode != GALX_ADJ_STEPS) return fail(GALX_E_ARG, "unknown adjoint storage %d", mode);
+    if (mode != c->adjStorage) c->trajValid = false;
+    c->adjStorage = mode;
     return GALX_OK;
 }
 
@@ -1300,46 +1423,20 @@ int galx_lserk_forward_ckpt(galx_ctx* c, double t, double dt, int nSteps, int ck
     if (nSteps < 1 || ckptEvery < 1 || !(dt == dt)) return fail(GALX_E_ARG, "bad step arguments");
     size_t ub = 5L * c->nElems * c->n3 * sizeof(double);
     int nSeg = (nSteps + ckptEvery - 1) / ckptEvery;
-    int nStage = 5 * std::min(ckptEvery, nSteps);
-    // storage: checkpoints, the stage inputs of one segment, the final state
-    while ((int)c->ckpt.size() > nSeg) {
-        dfree(c->ckpt.back());
-        c->bytes -= (long long)ub;
-        c->ckpt.pop_back();
-    }
-    while ((int)c->stages.size() > nStage) {
-        dfree(c->stages.back());
-        c->bytes -= (long long)ub;
-        c->stages.pop_back();
-    }
-    while ((int)c->ckpt.size() < nSeg) {
-        double* p = nullptr;
-        if ((rc = dalloc(c, &p, ub))) return fail(rc, "checkpoint storage (%d x %zu bytes): %s", nSeg, ub, g_err.c_str());
-        c->ckpt.push_back(p);
-    }
-    while ((int)c->stages.size() < nStage) {
-        double* p = nullptr;
-        if ((rc = dalloc(c, &p, ub)))
-            return fail(rc, "stage storage (%d x %zu bytes; lower ckptEvery): %s", nStage, ub, g_err.c_str());
-        c->stages.push_back(p);
-    }
-    // FP64 form: the recompute keeps every stage's trace records for the VJPs (the mixed form recomputes FP64
-    // records per VJP instead, its forward traces being FP32)
+    const int C = std::min(ckptEvery, nSteps);
+    const bool keepStages = c->adjStorage == GALX_ADJ_STAGES;
+    // storage: checkpoints; GALX_ADJ_STAGES: the stage inputs of one segment, and in the FP64 form their trace
+    // records, kept by the recompute for the VJPs (the mixed form recomputes FP64 records per VJP, its forward
+    // records being FP32); GALX_ADJ_STEPS: the step inputs of one segment and the stage inputs 1..4 of one step,
+    // trace records recomputed per VJP in both forms
     const size_t tb = 2L * 5 * c->nSides * c->n2 * sizeof(double);
-    const int nStageTr = c->tsz == 8 ? nStage : 0;
-    while ((int)c->stageTr.size() > nStageTr) {
-        dfree(c->stageTr.back());
-        c->bytes -= (long long)tb;
-        c->stageTr.pop_back();
-    }
-    while ((int)c->stageTr.size() < nStageTr) {
-        double* p = nullptr;
-        if ((rc = dalloc(c, &p, tb)))
-            return fail(rc, "stage trace storage (%d x %zu bytes; lower ckptEvery): %s", nStage, tb, g_err.c_str());
-        c->stageTr.push_back(p);
-    }
-    if ((rc = dalloc(c, &c->dUfinal, ub))) return rc;
     c->trajValid = false;
+    if ((rc = resize_buffers(c, c->ckpt, nSeg, ub, "checkpoint storage"))) return rc;
+    if ((rc = resize_buffers(c, c->stages, keepStages ? 5 * C : 4, ub, "stage storage"))) return rc;
+    if ((rc = resize_buffers(c, c->stageTr, keepStages && c->tsz == 8 ? 5 * C : 0, tb, "stage trace storage")))
+        return rc;
+    if ((rc = resize_buffers(c, c->steps, keepStages ? 0 : C - 1, ub, "step storage"))) return rc;
+    if ((rc = dalloc(c, &c->dUfinal, ub))) return rc;
     for (int seg = 0; seg < nSeg; seg++) {
         HIPCHK(hipMemcpyAsync(c->ckpt[seg], c->dU, ub, hipMemcpyDeviceToDevice, c->stream));
         int n = std::min(ckptEvery, nSteps - seg * ckptEvery);
@@ -1355,6 +1452,55 @@ int galx_lserk_forward_ckpt(galx_ctx* c, double t, double dt, int nSteps, int ck
     return GALX_OK;
 }
 
+static int adj_finish(galx_ctx* c, const double* pending);
+
+// one LSERK stage of the recorded trajectory recomputed from its input Ui into Uo, with the trace records
+// alternating between the context's two trace buffers (q: the stage's running index)
+static void adj_recompute_stage(galx_ctx* c, int step, int stage, int q, const double* Ui, double* Uo)
+{
+    const double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
+    const double ts = stage_time(step_time(c->trajT, c->trajDt, step), c->trajDt, stage);
+    launch_lift(c, Ui, c->dUtr[q & 1]);
+    launch_rhs(c, Stage{mRKA, RKB[stage] * c->trajDt, 1}, ts, Ui, Uo, c->dUtr[q & 1], c->dUtr[(q + 1) & 1]);
+}
+
+// reverse sweep with GALX_ADJ_STEPS storage: per segment, the recompute keeps the step inputs; per step, in
+// reverse, the stage inputs 1..4 are recomputed from the step input before the five stage VJPs, whose FP64 trace
+// records are recomputed too.  Same VJPs on bitwise the same stage inputs as the GALX_ADJ_STAGES sweep.
+static int adj_reverse_steps(galx_ctx* c)
+{
+    const int C = c->trajEvery, nSteps = c->trajSteps;
+    const int nSeg = (nSteps + C - 1) / C;
+    const double* pending = nullptr;
+    for (int seg = nSeg - 1; seg >= 0; seg--) {
+        const int s0 = seg * C, n = std::min(C, nSteps - s0);
+        auto step_in = [&](int s) -> double* { return s == 0 ? c->ckpt[seg] : c->steps[s - 1]; };
+        c->ks.trace(c->stream, c->nElems, c->ckpt[seg], mesh_of(c), c->dUtr[0], gas_of(c));
+        for (int s = 0; s + 1 < n; s++)
+            for (int stage = 0; stage < 5; stage++)
+                adj_recompute_stage(c, s0 + s, stage, 5 * s + stage, stage == 0 ? step_in(s) : c->stages[stage - 1],
+                                    stage == 4 ? step_in(s + 1) : c->stages[stage]);
+        HIPCHK(hipGetLastError());
+        for (int s = n - 1; s >= 0; s--) {
+            c->ks.trace(c->stream, c->nElems, step_in(s), mesh_of(c), c->dUtr[0], gas_of(c));
+            for (int stage = 0; stage < 4; stage++)
+                adj_recompute_stage(c, s0 + s, stage, stage, stage == 0 ? step_in(s) : c->stages[stage - 1],
+                                    c->stages[stage]);
+            for (int stage = 4; stage >= 0; stage--) {
+                const double* Us = stage == 0 ? step_in(s) : c->stages[stage - 1];
+                const double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
+                adj_apply(c, Us, adj_traces(c, Us, nullptr), c->dKbar, c->dUbar, 1, RKB[stage] * c->trajDt, 1, mRKA,
+                          pending, false);
+                // the pending state is read by the next VJP: a stage buffer of this step or a step input, neither
+                // overwritten before that
+                pending = Us;
+            }
+            HIPCHK(hipGetLastError());
+        }
+    }
+    return adj_finish(c, pending);
+}
+
 // reverse sweep over the recorded trajectory; dUbar (element planes) holds dJ/dU(final) on entry and
 // dJ/dU(initial) on exit.  The LSERK stage (src/timedisc/timestep.f90:85-119)
 //   k' = a k + R(U),  U' = U + b dt k'
@@ -1368,6 +1514,7 @@ static int adj_reverse(galx_ctx* c)
     GasD gd = gas_of(c);
     const double* pending = nullptr;  // state of the last VJP whose neighbour lifting term is still to be added
     HIPCHK(hipMemsetAsync(c->dKbar, 0, ub, c->stream));
+    if (c->adjStorage == GALX_ADJ_STEPS) return adj_reverse_steps(c);
     for (int seg = nSeg - 1; seg >= 0; seg--) {
         const int s0 = seg * C, n = std::min(C, nSteps - s0);
         // recompute the segment from its checkpoint; each stage reads its input and writes the next stage's input
@@ -1402,11 +1549,20 @@ static int adj_reverse(galx_ctx* c)
             }
         HIPCHK(hipGetLastError());
     }
+    return adj_finish(c, pending);
+}
+
+// after the last VJP of a sweep: its pending neighbour lifting term, then the context back at the final state of
+// the trajectory
+static int adj_finish(galx_ctx* c, const double* pending)
+{
+    size_t ub = 5L * c->nElems * c->n3 * sizeof(double);
+    Mesh m = mesh_of(c);
+    GasD gd = gas_of(c);
     if (pending) {
         TimerScope ts(c, "adjoint");
         c->as.face(c->stream, c->nElems, pending, m, c->dNghtr, adj_gas_of(c), c->dUbar);
     }
-    // back to the final state of the trajectory
     HIPCHK(hipMemcpyAsync(c->dU, c->dUfinal, ub, hipMemcpyDeviceToDevice, c->stream));
     c->cur = 0;
     c->ks.trace(c->stream, c->nElems, c->dU, m, c->dUtr[0], gd);
@@ -1633,17 +1789,17 @@ extern "C" int galx_prolong_to_face(int device, int nVar, int N, int nElems, int
     return GALX_OK;
 }
 
-extern "C" int galx_surf_int(int device, int nVar, int N, int nElems, int nSides, const int* S2E, const int* S2V2,
-                             const double* Fm, const double* Fs, double LHm, double LHp, double* Ut)
+// the host-side half of SurfIntCons for both entry points: argument checks, ElemToSide from SideToElem and
+// S2V2_inv from S2V2 (p + n*q for every flip, local side and volume face node)
+static int surf_int_prepare(int nVar, int N, int nElems, int nSides, const int* S2E, const int* S2V2, const double* Fm,
+                            const double* Fs, const double* Ut, std::vector<int>& E2S, std::vector<int>& inv)
 {
     if (nVar < 1 || nVar > 16 || N < 1 || N > 30 || nElems < 1 || nSides < 1 || !S2E || !S2V2 || !Fm || !Fs || !Ut)
         return fail(GALX_E_ARG, "bad arguments");
-    HIPCHK(hipSetDevice(device));
-    std::vector<int> E2S;
     int rc = build_e2s_from_s2e(nElems, nSides, S2E, E2S);
     if (rc) return rc;
     int n = N + 1, n2 = n * n;
-    std::vector<int> inv(5 * 6 * n2, -1);
+    inv.assign(5 * 6 * n2, -1);
     for (int ls = 1; ls <= 6; ls++)
         for (int f = 0; f <= 4; f++)
             for (int q = 0; q < n; q++)
@@ -1655,6 +1811,25 @@ extern "C" int galx_surf_int(int device, int nVar, int N, int nElems, int nSides
                 }
     for (int x : inv)
         if (x < 0) return fail(GALX_E_MESH, "S2V2 is not a bijection");
+    return GALX_OK;
+}
+
+static void launch_unit_surfint(int nVar, int N, int nElems, const int* dE, const int* dI, const double* dFm,
+                                const double* dFs, double LHm, double LHp, double* dUt)
+{
+    long nt = (long)nElems * (N + 1) * (N + 1) * (N + 1);
+    k_unit_surfint<<<dim3((unsigned)((nt + 127) / 128)), dim3(128)>>>(nVar, N, nElems, dE, dI, dFm, dFs, LHm, LHp,
+                                                                     dUt);
+}
+
+extern "C" int galx_surf_int(int device, int nVar, int N, int nElems, int nSides, const int* S2E, const int* S2V2,
+                             const double* Fm, const double* Fs, double LHm, double LHp, double* Ut)
+{
+    std::vector<int> E2S, inv;
+    int rc = surf_int_prepare(nVar, N, nElems, nSides, S2E, S2V2, Fm, Fs, Ut, E2S, inv);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(device));
+    int n = N + 1, n2 = n * n;
     size_t sv = (size_t)nVar * n * n * n * nElems * 8, sf = (size_t)nVar * n2 * nSides * 8;
     DevBuf dE, dI, dM, dSl, dU;
     HIPCHK(hipMalloc(&dE.p, E2S.size() * 4));
@@ -1667,10 +1842,8 @@ extern "C" int galx_surf_int(int device, int nVar, int N, int nElems, int nSides
     HIPCHK(hipMemcpy(dM.p, Fm, sf, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dSl.p, Fs, sf, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dU.p, Ut, sv, hipMemcpyHostToDevice));
-    long nt = (long)nElems * n * n * n;
-    k_unit_surfint<<<dim3((unsigned)((nt + 127) / 128)), dim3(128)>>>(nVar, N, nElems, (int*)dE.p, (int*)dI.p,
-                                                                     (double*)dM.p, (double*)dSl.p, LHm, LHp,
-                                                                     (double*)dU.p);
+    launch_unit_surfint(nVar, N, nElems, (int*)dE.p, (int*)dI.p, (double*)dM.p, (double*)dSl.p, LHm, LHp,
+                        (double*)dU.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(Ut, dU.p, sv, hipMemcpyDeviceToHost));

@@ -8,6 +8,7 @@
 //   utr[2]        : TT [nSides][2][5][n^2]           U traces, slot 0 master / 1 slave, master (p,q); double buffered
 //   vtr           : TT [nSides][2][4][n^2]           viscous normal-flux traces F_v(U,grad U).n_master, components
 //                                                    (rho u, rho v, rho w, E) of each side (the mass component is 0)
+//   geo32, sgeo32, fgeo32 : float, as geo, sgeo, fgeo  (mixed form only: their FP32 roundings)
 //   conn          : int [nElems][6]                  (SideID-1) << 3 | flip
 //   inv           : int [5 flips][6 locSides][n^2]   S2V2_inv as p + n*q
 // TT (trace type) is double in the FP64 form and float in the mixed form.
@@ -18,6 +19,8 @@
 // 0.5*(F_v(master) + F_v(slave)).n (riemann.f90:638-700) needs only the normal viscous flux of each side, so the
 // sides exchange those 4 values per face node instead of the 12 gradient components.
 #pragma once
+#include <type_traits>
+
 #include "galx_physics.hpp"
 
 #ifndef GALX_UNROLL_L
@@ -29,13 +32,24 @@
 #ifndef GALX_MIXED_WAVES
 #define GALX_MIXED_WAVES 6
 #endif
+// FP64 split form: the partner values of the flux differencing (rho,u,v,w,p,H and the direction's metric, all
+// halved) in LDS as five planes of double pairs, read with 16-byte loads, instead of nine planes of doubles
+#ifndef GALX_PAIR_PLANES
+#define GALX_PAIR_PLANES 1
+#endif
+#ifndef GALX_OWN_SLOT
+#define GALX_OWN_SLOT 1
+#endif
 
 namespace galx {
 
+// one workgroup per element: n^3 threads rounded up to whole waves, at most 1024 (N <= 9 gives every volume node a
+// thread of its own; the high-order kernels of N >= 10 loop their 1024 threads over the nodes)
 template <int N>
 struct Dm {
     static constexpr int n = N + 1, n2 = n * n, n3 = n2 * n, nf = 6 * n2;
-    static constexpr int block = n3 <= 64 ? 64 : ((n3 + 63) / 64) * 64;
+    static constexpr int block = n3 <= 64 ? 64 : (n3 <= 1024 ? ((n3 + 63) / 64) * 64 : 1024);
+    static constexpr bool node_per_thread = n3 <= block;
     static constexpr int ft = (nf + block - 1) / block;
 };
 
@@ -112,6 +126,30 @@ struct Mesh {
     const int* conn;
     const int* inv;
     const double* sgeo;  // [nElems][10][surface_count(N)]: geo at the surface nodes, in surface_node order
+    // the FP32 roundings of geo, fgeo and sgeo (mixed form only, else null).  The mixed kernels use every metric,
+    // face normal, tangent and SurfElem value as an FP32 operand; reading these copies gives them the same
+    // operands, rounded once by the host-side set-up instead of on every load, at half the bytes.  The FP64
+    // arrays stay the master copy: the Jacobian of the update, the time step and the adjoint read them.
+    const float* geo32;
+    const float* fgeo32;
+    const float* sgeo32;
+    void* ws;  // per-element workspace of the high-order kernels (N >= 10), else null
+};
+
+// the geometry arrays a kernel of compute type T reads its metric and face operands from
+template <typename T>
+struct GeoOf;
+template <>
+struct GeoOf<double> {
+    __device__ static __forceinline__ const double* geo(const Mesh& m) { return m.geo; }
+    __device__ static __forceinline__ const double* fgeo(const Mesh& m) { return m.fgeo; }
+    __device__ static __forceinline__ const double* sgeo(const Mesh& m) { return m.sgeo; }
+};
+template <>
+struct GeoOf<float> {
+    __device__ static __forceinline__ const float* geo(const Mesh& m) { return m.geo32; }
+    __device__ static __forceinline__ const float* fgeo(const Mesh& m) { return m.fgeo32; }
+    __device__ static __forceinline__ const float* sgeo(const Mesh& m) { return m.sgeo32; }
 };
 
 struct Stage {
@@ -202,8 +240,8 @@ __device__ __forceinline__ void lift_face_fluxes(int e, int tid, const Mesh& mes
         }
         trace_state(tr, gas, uc, pr);
         T Lnb[4] = {pr[1], pr[2], pr[3], pr[5]};
-        const double* fg = mesh.fgeo + side * 10 * n2 + pq;
-        T se = T(fg[9 * n2]);
+        const T* fg = GeoOf<T>::fgeo(mesh) + side * 10 * n2 + pq;
+        T se = fg[9 * n2];
         T* o = s_lflux + (ls - 1) * 7 * n2 + ab;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -211,9 +249,9 @@ __device__ __forceinline__ void lift_face_fluxes(int e, int tid, const Mesh& mes
             T Lm = flip ? Lnb[q] : own, Lsl = flip ? own : Lnb[q];
             o[q * n2] = T(0.5) * se * (T(-1) * Lm + Lsl);
         }
-        o[4 * n2] = T(fg[0]);
-        o[5 * n2] = T(fg[n2]);
-        o[6 * n2] = T(fg[2 * n2]);
+        o[4 * n2] = fg[0];
+        o[5 * n2] = fg[n2];
+        o[6 * n2] = fg[2 * n2];
     }
 }
 
@@ -223,9 +261,9 @@ template <int N, typename T>
 __device__ __forceinline__ void load_node_geo(int e, int node, const Mesh& mesh, T* gm)
 {
     constexpr int n3 = Dm<N>::n3;
-    const double* ge = mesh.geo + (long)e * 10 * n3 + node;
+    const T* ge = GeoOf<T>::geo(mesh) + (long)e * 10 * n3 + node;
 #pragma unroll
-    for (int q = 0; q < 10; q++) gm[q] = T(ge[q * n3]);
+    for (int q = 0; q < 10; q++) gm[q] = ge[q * n3];
 }
 
 // the same ten values of surface node t (node surface_node<N>(t)) from the surface-node copy: 10 * surface_count
@@ -234,9 +272,9 @@ template <int N, typename T>
 __device__ __forceinline__ void load_surf_geo(int e, int t, const Mesh& mesh, T* gm)
 {
     constexpr int ns = surface_count(N);
-    const double* ge = mesh.sgeo + (long)e * 10 * ns + t;
+    const T* ge = GeoOf<T>::sgeo(mesh) + (long)e * 10 * ns + t;
 #pragma unroll
-    for (int q = 0; q < 10; q++) gm[q] = T(ge[q * ns]);
+    for (int q = 0; q < 10; q++) gm[q] = ge[q * ns];
 }
 
 // gm: the node's Ja^1(3), Ja^2(3), Ja^3(3), sJ (load_node_geo), loaded by the caller ahead of the barrier that
@@ -449,6 +487,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (LiftWaves<N, T>::value))
            const T* __restrict__ mats, GasD gd, T LHm, T LHp)
 {
     using D = Dm<N>;
+    static_assert(D::node_per_thread, "one volume node per thread: N <= 9");
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
     constexpr int n3p = Lay<N>::n3;
     __shared__ T s_buf[4 * n3p + 6 * 12 * n2];  // own (u,v,w,T) | lifting fluxes, then own face gradients
@@ -502,8 +541,8 @@ __global__ void __launch_bounds__(Dm<N>::block, (LiftWaves<N, T>::value))
         long side = c >> 3;
         int flip = c & 7;
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
-        const double* fg = mesh.fgeo + side * 10 * n2 + pq;
-        T nv[3] = {T(fg[0]), T(fg[n2]), T(fg[2 * n2])};
+        const T* fg = GeoOf<T>::fgeo(mesh) + side * 10 * n2 + pq;
+        T nv[3] = {fg[0], fg[n2], fg[2 * n2]};
         const int pn = Lay<N>::at(node);
         T P[4] = {T(0), s_lift[pn], s_lift[n3p + pn], s_lift[2 * n3p + pn]};
         T G[12];
@@ -520,7 +559,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (LiftWaves<N, T>::value))
 
 // ---------------------------------------------------------------------------------------------------------
 // k_grad: the BR1 volume gradients of (u,v,w,T) of the current state, as the reference keeps them in
-// gradUx/y/z (lifting_br1.t90:47-167 with PP_OPTLIFT), for analysis: gvol [e][12][n3] in FP64
+// gradUx/y/z (lifting_br1.t90:47-167 with PP_OPTLIFT), for analysis: planes 0..11 of gvol [e][15][n3] in FP64
 // ---------------------------------------------------------------------------------------------------------
 template <int N, typename T, typename TT>
 __global__ void __launch_bounds__(Dm<N>::block)
@@ -528,6 +567,7 @@ __global__ void __launch_bounds__(Dm<N>::block)
            T LHm, T LHp, double* __restrict__ gvol)
 {
     using D = Dm<N>;
+    static_assert(D::node_per_thread, "one volume node per thread: N <= 9");
     constexpr int n2 = D::n2, n3 = D::n3;
     constexpr int n3p = Lay<N>::n3;
     __shared__ T s_buf[4 * n3p + 6 * 7 * n2];
@@ -548,9 +588,60 @@ __global__ void __launch_bounds__(Dm<N>::block)
     T g[12];
     lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_buf, s_buf + 4 * n3p, gas, LHm, LHp, g);
     if (tid < n3) {
-        double* o = gvol + (long)e * 12 * n3 + tid;
+        double* o = gvol + (long)e * 15 * n3 + tid;
 #pragma unroll
         for (int q = 0; q < 12; q++) o[q * n3] = double(g[q]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// k_grad_rho: the BR1 gradient of the density, the lifting variable LIFT_DENS that the reference lifts by default
+// (PP_OPTLIFT = 0, src/equations/navierstokes/idealgas/eos.h:133-142) and that no flux here needs.  The same
+// strong-form lifting as k_grad (lifting_br1.t90:47-167: sJ [sum_d Ja^d d/dxi_d rho + L_hat 0.5 (rho_slave -
+// rho_master) SurfElem n]), in FP64 from the FP64 state and the neighbours' trace records; analysis only, so it
+// reads its operands from memory directly.  Writes planes 12..14 (x, y, z) of gvol [e][15][n3].
+//   DT: D_T (0:N,0:N) column-major in FP64
+// ---------------------------------------------------------------------------------------------------------
+template <int N, typename TT>
+__global__ void __launch_bounds__(Dm<N>::block)
+    k_grad_rho(const double* __restrict__ U, Mesh mesh, const TT* __restrict__ utr, const double* __restrict__ DT,
+               double LHm, double LHp, double* __restrict__ gvol)
+{
+    using D = Dm<N>;
+    constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
+    const int e = blockIdx.x;
+    const double* rho = U + (long)e * 5 * n3;  // density plane of element e
+    const double* ge = mesh.geo + (long)e * 10 * n3;
+    for (int node = threadIdx.x; node < n3; node += D::block) {
+        const int i = node % n, j = (node / n) % n, k = node / n2;
+        double dxi = 0.0, det = 0.0, dze = 0.0;
+        for (int l = 0; l <= N; l++) {
+            dxi += DT[l + n * i] * rho[Lay<N>::idx(l, j, k)];
+            det += DT[l + n * j] * rho[Lay<N>::idx(i, l, k)];
+            dze += DT[l + n * k] * rho[Lay<N>::idx(i, j, l)];
+        }
+        double loc[3] = {0.0, 0.0, 0.0};
+        for (int ls = 1; ls <= 6; ls++) {
+            int ab;
+            if (!on_face<N>(ls, i, j, k, ab)) continue;
+            const int c = mesh.conn[e * 6 + ls - 1];
+            const long side = c >> 3;
+            const int flip = c & 7;
+            const int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
+            const double nb = double(utr[((side * 2 + (flip ? 0 : 1)) * 5) * n2 + pq]);  // neighbour's rho
+            const double own = rho[node];
+            const double Lm = flip ? nb : own, Lsl = flip ? own : nb;
+            const double* fg = mesh.fgeo + side * 10 * n2 + pq;
+            const double lf = 0.5 * fg[9 * n2] * (-1.0 * Lm + Lsl);
+            const double LH = (ls == 1 || ls == 2 || ls == 5) ? LHm : LHp;
+            for (int d = 0; d < 3; d++) loc[d] = loc[d] + (lf * fg[d * n2]) * LH;
+        }
+        const double sJ = ge[9 * n3 + node];
+        double* o = gvol + (long)e * 15 * n3 + node;
+        for (int d = 0; d < 3; d++) {
+            const double v = ge[d * n3 + node] * dxi + ge[(3 + d) * n3 + node] * det + ge[(6 + d) * n3 + node] * dze;
+            o[(12 + d) * n3] = sJ * (v + loc[d]);
+        }
     }
 }
 
@@ -569,24 +660,24 @@ __global__ void __launch_bounds__(Dm<N>::block)
 // ---------------------------------------------------------------------------------------------------------
 template <int N, typename T, typename TT, bool SPLIT, int RIEMANN>
 __device__ __forceinline__ void face_flux(const T* tro, const T* fvo, const T* trn, const T* fvn,
-                                          const double* __restrict__ fg, int n2, int flip, const Gas<T>& gas, T* F)
+                                          const T* __restrict__ fg, int n2, int flip, const Gas<T>& gas, T* F)
 {
     T Uo[5], Un[5], Po[6], Pn[6];
     trace_state(tro, gas, Uo, Po);
     trace_state(trn, gas, Un, Pn);
-    T nv[3] = {T(fg[0]), T(fg[n2]), T(fg[2 * n2])};
+    T nv[3] = {fg[0], fg[n2], fg[2 * n2]};
     const T *UL = flip ? Un : Uo, *UR = flip ? Uo : Un, *PL = flip ? Pn : Po, *PR = flip ? Po : Pn;
     if (SPLIT && RIEMANN == RIEMANN_LF) {
         lf_pi_cartesian<T>(UL, UR, PL, PR, nv, gas, F);
     } else {
-        T t1[3] = {T(fg[3 * n2]), T(fg[4 * n2]), T(fg[5 * n2])};
-        T t2[3] = {T(fg[6 * n2]), T(fg[7 * n2]), T(fg[8 * n2])};
+        T t1[3] = {fg[3 * n2], fg[4 * n2], fg[5 * n2]};
+        T t2[3] = {fg[6 * n2], fg[7 * n2], fg[8 * n2]};
         riemann_point<T, SPLIT>(RIEMANN, UL, UR, PL, PR, nv, t1, t2, gas, F);
     }
     // ViscousFlux (riemann.f90:638-700): 0.5*(F_v(master) + F_v(slave)).n
     const T* fvm = flip ? fvn : fvo;
     const T* fvs = flip ? fvo : fvn;
-    T se = T(fg[9 * n2]);
+    T se = fg[9 * n2];
     F[0] = F[0] * se;
 #pragma unroll
     for (int v = 1; v < 5; v++) F[v] = ((F[v] + T(0.5) * fvm[v - 1]) + T(0.5) * fvs[v - 1]) * se;
@@ -608,13 +699,20 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
           GasD gd, T LHm, T LHp, Stage st)
 {
     using D = Dm<N>;
+    static_assert(D::node_per_thread, "one volume node per thread: N <= 9");
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
     constexpr int np = Lay<N>::np, n3p = Lay<N>::n3;  // padded node layout of the line-read arrays
     constexpr int w_a = 4 * n3p + 6 * 7 * n2;         // lifting
     // mixed split form: the nine metric terms of every node stay in LDS (FP32, halves) in s_geo for the whole
     // kernel, so the volume terms read them there instead of from memory a second time (7.8 % faster at N=7)
     constexpr bool geo_lds = SPLIT && sizeof(T) == 4;
-    constexpr int w_c = SPLIT ? (geo_lds ? 10 : 13) * n3p : 5 * n3p;  // volume fluxes / split-form partner data
+    // FP64 split form: the partner data as pair planes (rho,u) (v,w) (p,H) (m0,m1) (m2,-), one 16-byte LDS read
+    // each, instead of nine 8-byte reads
+    constexpr bool pairs = SPLIT && sizeof(T) == 8 && GALX_PAIR_PLANES;
+    // mixed form: the face tasks read the own trace record from the trace buffer's own slot (written from the same
+    // state by the previous stage or by k_trace, so bitwise the record U would give)
+    constexpr bool own_slot = sizeof(TT) == 4 && GALX_OWN_SLOT;
+    constexpr int w_c = SPLIT ? (geo_lds ? 10 : (pairs ? 14 : 13)) * n3p : 5 * n3p;  // volume fluxes / partners
     constexpr int w_d = 6 * 5 * n2;  // face results, new traces
     // split form: the i- and j-lines lie inside one 64-lane wave (n and n^2 divide 64: N = 1, 3, 7), so the pairs of
     // those lines are shared between lanes instead of formed twice
@@ -667,14 +765,15 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     __syncthreads();
 
     T ut[5] = {0, 0, 0, 0, 0};
-    const double* ge = mesh.geo + (long)e * 10 * n3 + tid;
+    const double* ge = mesh.geo + (long)e * 10 * n3 + tid;  // FP64 sJ of the update
+    const T* geT = GeoOf<T>::geo(mesh) + (long)e * 10 * n3 + tid;
     if (!SPLIT) {
         // ---- full transformed fluxes, weak-form D_Hat_T contraction (VolInt_weakForm) ----
         T* s_f = s_w;  // [5][n3p]
 #pragma unroll 1
         for (int d = 0; d < 3; d++) {
             if (active) {
-                T m[3] = {T(ge[(3 * d) * n3]), T(ge[(3 * d + 1) * n3]), T(ge[(3 * d + 2) * n3])};
+                T m[3] = {geT[(3 * d) * n3], geT[(3 * d + 1) * n3], geT[(3 * d + 2) * n3]};
                 T f[5];
                 transformed_flux<T, true>(uc, pr, s, m, gas.hshift, f);
 #pragma unroll
@@ -702,13 +801,34 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         // (rho,u,v,w,p,H) and metric, and the sum is doubled once at the end (exact)
         T* s_v = s_w;                                // [6][n3p] rho,u,v,w,p,H (halves)
         T* s_m = s_w + 6 * n3p;                      // [3][n3p] metric of the current direction (halves)
-        T* s_f = s_w + (geo_lds ? 6 : 9) * n3p;      // [4][n3p] viscous contravariant flux (momentum, energy) of
-                                                     // the current direction
+        T* s_f = s_w + (geo_lds ? 6 : (pairs ? 10 : 9)) * n3p;  // [4][n3p] viscous contravariant flux (momentum,
+                                                                // energy) of the current direction
+        // pairs: [5][n3p] pair planes in place of s_v and s_m
+        using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+        T2* s_p = reinterpret_cast<T2*>(s_w);
+        // the halved state and metric of partner node o
+        auto partner = [&](const T* sv, const T* sm, const T2* sp, int o, T* Po, T* Mo) {
+            if (pairs) {
+                const T2 a = sp[o], b = sp[n3p + o], c = sp[2 * n3p + o], dm = sp[3 * n3p + o], f = sp[4 * n3p + o];
+                Po[0] = a.x; Po[1] = a.y; Po[2] = b.x; Po[3] = b.y; Po[4] = c.x; Po[5] = c.y;
+                Mo[0] = dm.x; Mo[1] = dm.y; Mo[2] = f.x;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 6; q++) Po[q] = sv[q * n3p + o];
+#pragma unroll
+                for (int c = 0; c < 3; c++) Mo[c] = sm[c * n3p + o];
+            }
+        };
         const T hv[6] = {T(0.5) * pr[0], T(0.5) * pr[1], T(0.5) * pr[2], T(0.5) * pr[3], T(0.5) * pr[4],
                          T(0.5) * Hme};
         if (active) {
+            if (pairs) {
 #pragma unroll
-            for (int q = 0; q < 6; q++) s_v[q * n3p + pt] = hv[q];
+                for (int q = 0; q < 3; q++) s_p[q * n3p + pt] = T2{hv[2 * q], hv[2 * q + 1]};
+            } else {
+#pragma unroll
+                for (int q = 0; q < 6; q++) s_v[q * n3p + pt] = hv[q];
+            }
         }
         T tmp[5] = {0, 0, 0, 0, 0};
 #pragma unroll 1
@@ -728,7 +848,11 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                         // the prefetched values, kept in registers since the load (geo_reg)
                         mme[c] = d == 0 ? gm[c] : (d == 1 ? gm[3 + c] : gm[6 + c]);
                         hm[c] = T(0.5) * mme[c];
-                        s_m[c * n3p + pt] = hm[c];
+                        if (!pairs) s_m[c * n3p + pt] = hm[c];
+                    }
+                    if (pairs) {
+                        s_p[3 * n3p + pt] = T2{hm[0], hm[1]};
+                        s_p[4 * n3p + pt] = T2{hm[2], T(0)};
                     }
                 }
                 T f[5];
@@ -762,11 +886,11 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                     for (int s = 1; s <= n / 2; s++) {
                         const int b = (idx + s) & msk;          // partner formed here
                         const int o = ln(b);
-                        T Mo[3] = {s_m[o], s_m[n3p + o], s_m[2 * n3p + o]};
+                        T Mo[3], Po[6];
+                        partner(s_v, s_m, s_p, o, Po, Mo);
                         T G[5];
-                        split_volume_flux_pi_half<T>(hv[0], hv[1], hv[2], hv[3], hv[4], hv[5], s_v[o],
-                                                     s_v[n3p + o], s_v[2 * n3p + o], s_v[3 * n3p + o],
-                                                     s_v[4 * n3p + o], s_v[5 * n3p + o], hm, Mo, G);
+                        split_volume_flux_pi_half<T>(hv[0], hv[1], hv[2], hv[3], hv[4], hv[5], Po[0], Po[1], Po[2],
+                                                     Po[3], Po[4], Po[5], hm, Mo, G);
                         const T db = dp[s * n];
                         if (2 * s < n) {
                             const int src = lane0 + ((idx - s) & msk) * st;  // the lane that formed (a-s, a)
@@ -789,11 +913,11 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll GALX_UNROLL_SPLIT
                 for (int l = 0; l <= N; l++) {
                     const int o = ln(l);
-                    T Mo[3] = {s_m[o], s_m[n3p + o], s_m[2 * n3p + o]};
+                    T Mo[3], Po[6];
+                    partner(s_v, s_m, s_p, o, Po, Mo);
                     T G[5];
-                    split_volume_flux_pi_half<T>(hv[0], hv[1], hv[2], hv[3], hv[4], hv[5], s_v[o], s_v[n3p + o],
-                                                 s_v[2 * n3p + o], s_v[3 * n3p + o], s_v[4 * n3p + o],
-                                                 s_v[5 * n3p + o], hm, Mo, G);
+                    split_volume_flux_pi_half<T>(hv[0], hv[1], hv[2], hv[3], hv[4], hv[5], Po[0], Po[1], Po[2],
+                                                 Po[3], Po[4], Po[5], hm, Mo, G);
                     T dv = M.dvs(l, idx);
 #pragma unroll
                     for (int v = 0; v < 5; v++) tmp[v] = tmp[v] + dv * G[v];
@@ -817,17 +941,24 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
         T Uo[5], fvo[4], Un[5], fvn[4];
         {
-            // from U at the face node, not from the own slot of the trace buffer: the same record, but that read
-            // goes to memory (measured 1.5 % slower in FP64 on MI355X).  Formed by the node threads from the state
-            // they hold and passed through LDS instead, it measured 6 % slower in FP64 at N=7: the state then
-            // stays live through the volume terms and spills.
-            const double* uo = U + (long)e * 5 * n3 + node;
-            double c[5], tr[5];
+            // FP64: from U at the face node, not from the own slot of the trace buffer: the same record, but that
+            // read goes to memory (measured 1.5 % slower in FP64 on MI355X).  Formed by the node threads from the
+            // state they hold and passed through LDS instead, it measured 6 % slower in FP64 at N=7: the state
+            // then stays live through the volume terms and spills.  Mixed form: the own slot, whose FP32 record is
+            // half the bytes of the FP64 state at the node (GALX_OWN_SLOT)
+            if (own_slot) {
+                const TT* to = utr + ((side * 2 + (flip ? 1 : 0)) * 5) * n2 + pq;
 #pragma unroll
-            for (int v = 0; v < 5; v++) c[v] = uo[v * n3];
-            trace_record<sizeof(TT) == 4>(c, gd.kappa, gd.pref, tr);
+                for (int v = 0; v < 5; v++) Uo[v] = T(to[v * n2]);
+            } else {
+                const double* uo = U + (long)e * 5 * n3 + node;
+                double c[5], tr[5];
 #pragma unroll
-            for (int v = 0; v < 5; v++) Uo[v] = T(TT(tr[v]));  // exactly the record the neighbour reads
+                for (int v = 0; v < 5; v++) c[v] = uo[v * n3];
+                trace_record<sizeof(TT) == 4>(c, gd.kappa, gd.pref, tr);
+#pragma unroll
+                for (int v = 0; v < 5; v++) Uo[v] = T(TT(tr[v]));  // exactly the record the neighbour reads
+            }
         }
         const TT* vo = vtr + ((side * 2 + (flip ? 1 : 0)) * 4) * n2 + pq;  // written by k_lift of this element
 #pragma unroll
@@ -840,7 +971,8 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll
         for (int q = 0; q < 4; q++) fvn[q] = T(vn[q * n2]);
         T F[5];
-        face_flux<N, T, TT, SPLIT, RIEMANN>(Uo, fvo, Un, fvn, mesh.fgeo + side * 10 * n2 + pq, n2, flip, gas, F);
+        face_flux<N, T, TT, SPLIT, RIEMANN>(Uo, fvo, Un, fvn, GeoOf<T>::fgeo(mesh) + side * 10 * n2 + pq, n2, flip,
+                                            gas, F);
         T sg = flip ? T(-1) : T(1);
 #pragma unroll
         for (int v = 0; v < 5; v++) s_res[(ls - 1) * 5 * n2 + v * n2 + ab] = sg * F[v];
@@ -927,9 +1059,9 @@ __global__ void __launch_bounds__(Dm<N>::block)
     const int e = blockIdx.x, tid = threadIdx.x;
     double lam[6] = {0, 0, 0, 0, 0, 0};
     int bad = 0;
-    if (tid < n3) {
-        const double* u = U + (long)e * 5 * n3 + tid;
-        const double* ge = geo + (long)e * 10 * n3 + tid;
+    for (int node = tid; node < n3; node += D::block) {
+        const double* u = U + (long)e * 5 * n3 + node;
+        const double* ge = geo + (long)e * 10 * n3 + node;
         double rho = u[0], srho = 1.0 / rho;
         double vel[3] = {u[n3] * srho, u[2 * n3] * srho, u[3 * n3] * srho};
         double p = (kappa - 1.0) * (u[4 * n3] - 0.5 * (vel[0] * u[n3] + vel[1] * u[2 * n3] + vel[2] * u[3 * n3]));
@@ -944,8 +1076,9 @@ __global__ void __launch_bounds__(Dm<N>::block)
             double lc = fabs(m0 * vsJ[0] + m1 * vsJ[1] + m2 * vsJ[2]) + c * adv;
             double a0 = m0 * sj, a1 = m1 * sj, a2 = m2 * sj;
             double lv = mu0 * srho * (KPmax * (a0 * a0 + a1 * a1 + a2 * a2));
-            lam[d] = lc;
-            lam[3 + d] = lv;
+            // running maximum over the thread's nodes, NaN-propagating (one node per thread for N <= 9)
+            lam[d] = (lc > lam[d] || lc != lc || node == tid) ? lc : lam[d];
+            lam[3 + d] = (lv > lam[3 + d] || lv != lv || node == tid) ? lv : lam[3 + d];
         }
     }
     // wave reduction (max, NaN-propagating)

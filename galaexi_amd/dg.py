@@ -151,12 +151,14 @@ class DGOperator:
         """Stage iStage (1..5) of TimeStepByLSERKW2."""
         check(lib().galx_lserk_stage(self._h, float(t), float(dt), int(iStage)), "galx_lserk_stage")
 
-    def get_gradients(self):
-        """Lifted gradients (gradUx, gradUy, gradUz) of the current state, each (4,0:N,0:N,0:N,nElems) over the
-        lifting variables (u,v,w,T)."""
+    def get_gradients(self, nVarLifting=4):
+        """Lifted gradients (gradUx, gradUy, gradUz) of the current state, each (nVarLifting,0:N,0:N,0:N,nElems)
+        over the lifting variables: (u,v,w,T) for nVarLifting=4 (PP_OPTLIFT), (rho,u,v,w,T) for 5 (the
+        reference's default lifting)."""
         n = self.n
-        out = [np.zeros((4, n, n, n, self.nElems), order="F") for _ in range(3)]
-        check(lib().galx_get_gradients(self._h, *[_dptr(a) for a in out]), "galx_get_gradients")
+        out = [np.zeros((nVarLifting, n, n, n, self.nElems), order="F") for _ in range(3)]
+        check(lib().galx_get_lifted_gradients(self._h, int(nVarLifting), *[_dptr(a) for a in out]),
+              "galx_get_lifted_gradients")
         return tuple(out)
 
     # CALCTIMESTEP
@@ -176,6 +178,14 @@ class DGOperator:
 
     def rhs_vjp_device(self, lam_ptr, ubar_ptr):
         check(lib().galx_rhs_vjp_device(self._h, C.c_void_p(lam_ptr), C.c_void_p(ubar_ptr)), "galx_rhs_vjp_device")
+
+    def set_adjoint_storage(self, mode):
+        """What the reverse sweep keeps between checkpoints: "stages" (each segment's stage inputs; fastest) or
+        "steps" (each segment's step inputs, the stages of a step recomputed before its VJPs; least memory)."""
+        modes = {"stages": _lib.GALX_ADJ_STAGES, "steps": _lib.GALX_ADJ_STEPS}
+        if mode not in modes:
+            raise ValueError(f"adjoint storage {mode!r}: one of {sorted(modes)}")
+        check(lib().galx_set_adjoint_storage(self._h, modes[mode]), "galx_set_adjoint_storage")
 
     def forward_ckpt(self, t, dt, nSteps, ckptEvery):
         """TimeStepByLSERKW2 over nSteps with the trajectory recorded for lserk_adjoint."""

@@ -115,6 +115,12 @@ int galx_lserk_stage(galx_ctx *ctx, double t, double dt, int iStage);
  * (src/equations/navierstokes/idealgas/eos.h:145-152): gradUx/y/z(4,0:N,0:N,0:N,nElems), FP64, host / device. */
 int galx_get_gradients(galx_ctx *ctx, double *gradUx, double *gradUy, double *gradUz);
 int galx_get_gradients_device(galx_ctx *ctx, double *dGradUx, double *dGradUy, double *dGradUz);
+/* The same for either lifting-variable set of the reference (eos.h:133-152): nVarLifting = 5 is its default
+ * (PP_OPTLIFT = 0), gradUx/y/z(5,0:N,0:N,0:N,nElems) over (rho,u,v,w,T), the density gradient lifted with the same
+ * BR1 operator (FP64); nVarLifting = 4 is galx_get_gradients. */
+int galx_get_lifted_gradients(galx_ctx *ctx, int nVarLifting, double *gradUx, double *gradUy, double *gradUz);
+int galx_get_lifted_gradients_device(galx_ctx *ctx, int nVarLifting, double *dGradUx, double *dGradUy,
+                                     double *dGradUz);
 
 /* CALCTIMESTEP(errType) (src/equations/navierstokes/calctimestep.f90:98-186), FP64 reduction.
  * errType: 0 ok, 1 non-positive or NaN density, 2 convective / 3 viscous time step is NaN. */
@@ -141,9 +147,21 @@ int galx_rhs_vjp(galx_ctx *ctx, const double *lam, double *Ubar);
 int galx_rhs_vjp_device(galx_ctx *ctx, const double *dLam, double *dUbar);
 /* galx_lserk_forward_ckpt: TimeStepByLSERKW2 for nSteps steps of fixed dt from the current state, recording
  * the trajectory for the reverse sweep: the state every ckptEvery steps is kept on the device, together with
- * room for the 5*min(ckptEvery,nSteps) stage inputs of one segment (device memory ~ (nSteps/ckptEvery +
- * 5*ckptEvery + 1) copies of U).  The context ends at the final state. */
+ * room for what the sweep recomputes between checkpoints (galx_set_adjoint_storage).  The context ends at the
+ * final state. */
 int galx_lserk_forward_ckpt(galx_ctx *ctx, double t, double dt, int nSteps, int ckptEvery);
+/* galx_set_adjoint_storage: what the reverse sweep keeps between checkpoints, with C = min(ckptEvery, nSteps).
+ *  GALX_ADJ_STAGES (default): each segment is recomputed once and its 5*C stage inputs are kept (in the FP64 form
+ *    with their trace records): device memory ~ (nSteps/ckptEvery + 5*C + 1) copies of U (FP64 form: plus 5*C
+ *    trace-record sets of 0.75 U each at N=7).
+ *  GALX_ADJ_STEPS: each segment is recomputed once keeping its C step inputs, and each step's stages are
+ *    recomputed before its VJPs (4 more RHS evaluations per step, FP64 trace records recomputed per VJP):
+ *    ~ (nSteps/ckptEvery + C + 5) copies of U.  The same VJPs on the same stage inputs: the gradient is bitwise
+ *    the GALX_ADJ_STAGES one.
+ * Changing the mode discards a recorded trajectory.  No counterpart in the reference (which has no adjoint). */
+#define GALX_ADJ_STAGES 0
+#define GALX_ADJ_STEPS 1
+int galx_set_adjoint_storage(galx_ctx *ctx, int mode);
 /* galx_lserk_adjoint: reverse sweep over the recorded trajectory (segments recomputed from their
  * checkpoints).  Ubar holds dJ/dU(final) on entry and dJ/dU(initial) on exit (U layout, host / device).
  * The context is left at the final state of the trajectory, which stays recorded.  galx_set_state[_device],

@@ -10,6 +10,7 @@ IMPLICIT NONE
 INTEGER(C_INT),PARAMETER :: GALX_FORM_WEAK=0, GALX_FORM_SPLIT_PI=1
 INTEGER(C_INT),PARAMETER :: GALX_RIEMANN_LF=0, GALX_RIEMANN_ROE=1, GALX_RIEMANN_ROEEF=3
 INTEGER(C_INT),PARAMETER :: GALX_PREC_FP64=0, GALX_PREC_MIXED=1
+INTEGER(C_INT),PARAMETER :: GALX_ADJ_STAGES=0, GALX_ADJ_STEPS=1
 TYPE,BIND(C) :: galx_config
   INTEGER(C_INT) :: N, form, riemann, precision
   REAL(C_DOUBLE) :: kappa, R, Pr, mu0, CFLScale, DFLScale
@@ -93,6 +94,18 @@ INTERFACE
     TYPE(C_PTR),VALUE    :: ctx
     REAL(C_DOUBLE),VALUE :: t, dt
     INTEGER(C_INT),VALUE :: nSteps, ckptEvery
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION galx_get_lifted_gradients(ctx,nVarLifting,gradUx,gradUy,gradUz) &
+      BIND(C,NAME='galx_get_lifted_gradients')
+    IMPORT :: C_INT, C_PTR, C_DOUBLE
+    TYPE(C_PTR),VALUE          :: ctx
+    INTEGER(C_INT),VALUE       :: nVarLifting
+    REAL(C_DOUBLE),INTENT(OUT) :: gradUx(*), gradUy(*), gradUz(*)
+  END FUNCTION
+  INTEGER(C_INT) FUNCTION galx_set_adjoint_storage(ctx,mode) BIND(C,NAME='galx_set_adjoint_storage')
+    IMPORT :: C_INT, C_PTR
+    TYPE(C_PTR),VALUE    :: ctx
+    INTEGER(C_INT),VALUE :: mode
   END FUNCTION
   INTEGER(C_INT) FUNCTION galx_lserk_adjoint(ctx,Ubar) BIND(C,NAME='galx_lserk_adjoint')
     IMPORT :: C_INT, C_PTR, C_DOUBLE

@@ -188,3 +188,55 @@ def test_riemann_flux_at_stationary_shock(riemann):
             assert np.allclose(F, exact - 0.5 * lmax * (UR - UL), rtol=1e-12, atol=1e-12)
         checked["up" if admissible else "down"] += 1
     assert checked["same"] > 0 and checked["up"] > 0 and checked["down"] > 0
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_lax_friedrichs_flux_jacobian_eigenvalues(split):
+    """Riemann_LF (src/equations/navierstokes/riemann.f90:707-733) is the local Lax-Friedrichs (Rusanov) flux
+    F*(U_L,U_R) = C(U_L,U_R) - lmax/2 (U_R - U_L), lmax = max(|u_n| + c) of the two sides, with C the central flux
+    (weak form: the mean of the Euler fluxes; split form: the Pirozzoli flux, splitflux.f90:735-761).  Any
+    consistent symmetric C has dC/dU_R = dC/dU_L = A/2 at U_L = U_R = U (A = dF(U).n/dU), and the jump term adds
+    -+lmax/2 I there, so the Jacobians of the oracle's face flux at a uniform state have the eigenvalues
+    (a_k -+ lmax)/2 with a_k = u_n - c, u_n (three times), u_n + c.  A wrong dissipation coefficient, sign or wave
+    speed moves them.  The Jacobian comes from central differences of the whole operator's master-side flux on a
+    two-element-wide box whose second element column carries the perturbed state."""
+    kappa = 1.4
+    m = M.box_mesh((2, 1, 1), 2, lo=(0, 0, 0), hi=(2, 1, 1))
+    xi = (m.Elem_xGP[0].reshape(-1, m.nElems, order="F").mean(axis=0) > 1.0).astype(int)
+    rho, vel, p = 1.1, np.array([0.35, -0.2, 0.15]), 0.8
+    base = T.freestream(m.Elem_xGP, rho=rho, vel=tuple(vel), p=p, kappa=kappa)
+    U0 = base[:, 0, 0, 0, 0].copy()
+    op = O.Operator(m, split=split, riemann=O.RIEMANN_LF, kappa=kappa, mu0=0.0, R=1.0)
+    # an x-face with master element in column 0, slave in column 1 (normal +x)
+    sides = [s for s in range(m.nSides)
+             if xi[m.SideToElem[0, s] - 1] == 0 and xi[m.SideToElem[1, s] - 1] == 1 and m.NormVec[0, 1, 1, s] > 0.5]
+    assert sides
+    s = sides[0]
+    nv = m.NormVec[:, 1, 1, s]
+    se = m.SurfElem[1, 1, s]
+
+    def face_flux(dL, dR):
+        U = base.copy(order="F")
+        for e in range(m.nElems):
+            U[:, :, :, :, e] += (dL if xi[e] == 0 else dR)[:, None, None, None]
+        op.rhs(U)
+        return op.array(8).reshape((5, 3, 3, m.nSides), order="F")[:, 1, 1, s] / se
+
+    eps = 1e-6
+    JL, JR = np.zeros((5, 5)), np.zeros((5, 5))
+    z = np.zeros(5)
+    for k in range(5):
+        d = np.zeros(5)
+        d[k] = eps * max(abs(U0[k]), 1.0)
+        JR[:, k] = (face_flux(z, d) - face_flux(z, -d)) / (2 * d[k])
+        JL[:, k] = (face_flux(d, z) - face_flux(-d, z)) / (2 * d[k])
+    un = vel @ nv
+    c = np.sqrt(kappa * p / rho)
+    lmax = abs(un) + c
+    a = np.array([un - c, un, un, un, un + c])
+    eR = np.sort(np.linalg.eigvals(JR).real)
+    eL = np.sort(np.linalg.eigvals(JL).real)
+    assert np.allclose(eR, np.sort((a - lmax) / 2), atol=1e-6 * lmax), (eR, (a - lmax) / 2)
+    assert np.allclose(eL, np.sort((a + lmax) / 2), atol=1e-6 * lmax), (eL, (a + lmax) / 2)
+    # and their sum is the flux Jacobian A itself (the dissipation cancels), whose eigenvalues are a_k
+    assert np.allclose(np.sort(np.linalg.eigvals(JL + JR).real), np.sort(a), atol=1e-6 * lmax)

@@ -287,6 +287,44 @@ def test_lserk_adjoint_checkpoint_spacing_invariant():
     op.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lserk_adjoint_step_storage_equals_stage_storage(precision):
+    """GALX_ADJ_STEPS keeps a segment's step inputs and recomputes each step's stages (and, in both forms, the
+    FP64 trace records) before its VJPs: the same VJPs on the same stage inputs, so the gradient is bitwise the
+    GALX_ADJ_STAGES one, for segments of several steps, of one step and a short last segment, with less device
+    memory; the context ends at the final state, and a mode change discards the recording."""
+    from galaexi_amd._lib import GalxError
+
+    say(f"case: adjoint step storage vs stage storage, {precision}, (3,3,2) N=3, 12 steps, ckpt 5/1/12")
+    m, U0 = lserk_case()
+    op = dgmod().DGOperator(3, form="split", riemann="lf", precision=precision, **GAS)
+    op.set_mesh(m)
+    op.set_state(U0)
+    dt, _ = op.CalcTimeStep()
+    rng = np.random.default_rng(3)
+    Ubar_final = np.asfortranarray(rng.standard_normal(U0.shape))
+    for every in (5, 1, 12):
+        res, mem = {}, {}
+        for mode in ("stages", "steps"):
+            op.set_adjoint_storage(mode)
+            op.set_state(U0)
+            op.forward_ckpt(0.0, dt, 12, every)
+            UN = op.get_state()
+            mem[mode] = op.memory_bytes()
+            res[mode] = op.lserk_adjoint(Ubar_final)
+            assert np.array_equal(op.get_state(), UN)
+        say(f"  ckpt {every}: device memory stages {mem['stages'] / 1e6:.1f} MB, steps {mem['steps'] / 1e6:.1f} MB")
+        assert np.array_equal(res["steps"], res["stages"])
+        if every > 1:
+            assert mem["steps"] < mem["stages"]
+    op.set_adjoint_storage("stages")
+    with pytest.raises(GalxError, match="trajectory"):
+        op.lserk_adjoint(Ubar_final)
+    with pytest.raises(ValueError):
+        op.set_adjoint_storage("none")
+    op.close()
+
+
 def test_autograd_wrappers():
     """torch.autograd wrappers: gradcheck (fast mode: random projections u^T J v against finite differences) of
     the RHS on a small mesh, and the LSERK wrapper's gradient equal to the direct ABI result."""

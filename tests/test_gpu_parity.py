@@ -109,6 +109,19 @@ CASES = [
     ((2, 2, 2), 7, "cart", "split", "lf"),
     ((2, 2, 2), 7, "curvrot", "split", "roeentropyfix"),
     ((2, 2, 2), 7, "curved", "weak", "roe"),
+    # the other one-node-per-thread degrees
+    ((2, 2, 2), 1, "curvrot", "split", "lf"),
+    ((2, 2, 2), 2, "curved", "weak", "roeentropyfix"),
+    ((2, 2, 2), 4, "rot", "weak", "lf"),
+    ((2, 2, 2), 6, "curvrot", "split", "roe"),
+    ((2, 2, 2), 8, "curved", "split", "lf"),
+    ((2, 2, 2), 9, "curvrot", "split", "lf"),
+    # the high-order kernels (N >= 10: more volume nodes than workgroup threads)
+    ((2, 2, 2), 10, "curvrot", "split", "roeentropyfix"),
+    ((2, 2, 2), 11, "curved", "weak", "roeentropyfix"),
+    ((2, 2, 2), 12, "rot", "split", "lf"),
+    ((2, 2, 2), 13, "curvrot", "weak", "roe"),
+    ((2, 2, 2), 13, "curvrot", "split", "lf"),
 ]
 
 
@@ -122,7 +135,7 @@ def test_rhs_fp64_vs_oracle(nel, N, kind, form, riemann):
         assert rel_l2(Ut[v], Ut_ref[v]) <= 1e-12, (v, rel_l2(Ut[v], Ut_ref[v]))
 
 
-@pytest.mark.parametrize("N,form", [(3, "split"), (7, "split"), (5, "weak")])
+@pytest.mark.parametrize("N,form", [(3, "split"), (7, "split"), (5, "weak"), (11, "split"), (13, "weak")])
 def test_freestream_curved_gpu(N, form):
     """Free-stream preservation on the curved, rotated periodic mesh: |Ut| at round-off."""
     say(f"case: freestream N={N} {form}")
@@ -135,14 +148,17 @@ def test_freestream_curved_gpu(N, form):
     Ut = op.DGTimeDerivative_weakForm()
     op.close()
     ref = O.Operator(m, split=(form == "split"), riemann=O.RIEMANN_ROEEF, mu0=1e-3, R=1.0).rhs(U)
-    # round-off scale of the operator: sJ * |U| * (N+1)^2 (line sums of D entries of size ~N^2)
+    # round-off scale of the operator: sJ * |U| * (N+1)^2 (line sums of D entries of size ~N^2).  From N = 10 on the
+    # round-off of the host-computed curl metrics grows faster than that: the oracle's own residual is 1.2e-9 at
+    # N = 11 and 1.7e-8 at N = 13 (3e-13 and 3e-12 of this scale), so the bar there is 1e-11 of it
     scale = np.abs(m.sJ).max() * np.abs(U).max() * (N + 1) ** 2
-    assert np.abs(Ut).max() <= 1e-13 * scale
+    assert np.abs(Ut).max() <= (1e-13 if N <= 9 else 1e-11) * scale
     # same order as the FP64 oracle's own residual
     assert np.abs(Ut).max() <= 30 * max(np.abs(ref).max(), 1e-15 * scale)
 
 
-@pytest.mark.parametrize("N,form,riemann", [(3, "weak", "roe"), (5, "split", "lf"), (7, "split", "roeentropyfix")])
+@pytest.mark.parametrize("N,form,riemann", [(3, "weak", "roe"), (5, "split", "lf"), (7, "split", "roeentropyfix"),
+                                            (9, "weak", "lf"), (10, "split", "lf"), (13, "split", "roeentropyfix")])
 def test_lserk_and_dt_vs_oracle(N, form, riemann):
     """Two TimeStepByLSERKW2 steps with dt from CALCTIMESTEP, against the oracle."""
     say(f"case: lserk N={N} {form} {riemann}")

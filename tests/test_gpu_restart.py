@@ -5,7 +5,9 @@ exactly as the uninterrupted run.
 Bar: FP64, bit-identical final state (the state file holds U exactly, the first LSERK stage does not read the
 register, A_1 = 0, and the traces the restarted context forms from U are the records the uninterrupted kernels form
 from the same values).  Mixed form: the restart recomputes the pressure offset of the FP32 arithmetic from the
-restart state, so the continuation agrees to the mixed form's own accuracy, 1e-9 relative here, not bitwise.
+restart state (the mean pressure has moved since t=0), so the continuation is not bitwise the uninterrupted one.  It
+must agree with it at least as closely as the uninterrupted mixed run agrees with the FP64 run over the same steps:
+the restart adds no error beyond the form's own.
 """
 import os
 
@@ -33,8 +35,17 @@ def op_for(m, precision):
     return op
 
 
-@pytest.mark.parametrize("precision,bar", [("fp64", 0.0), ("mixed", 1e-9)])
-def test_write_restart_continue(tmp_path, precision, bar):
+def run_uninterrupted(m, U0, precision, dt, steps):
+    op = op_for(m, precision)
+    op.set_state(U0)
+    op.TimeStepByLSERKW2(0.0, dt, steps)
+    U = op.get_state()
+    op.close()
+    return U
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_write_restart_continue(tmp_path, precision):
     say(f"case: write state / restart / continue, curved TGV 3^3 N=7, {precision}")
     m = M.box_mesh((3, 3, 3), 7, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3, deform_amp=0.05,
                    deform_phase=(0.31, 0.17, 0.83))
@@ -58,7 +69,10 @@ def test_write_restart_continue(tmp_path, precision, bar):
     op.close()
     rel = np.linalg.norm((U - U_ref).ravel()) / np.linalg.norm(U_ref.ravel())
     say(f"  restarted vs uninterrupted after {k1}+{k2} steps: rel L2 {rel:.2e}")
-    if bar == 0.0:
+    if precision == "fp64":
         assert np.array_equal(U, U_ref)
     else:
-        assert rel <= bar
+        U64 = run_uninterrupted(m, U0, "fp64", dt, k1 + k2)
+        form = np.linalg.norm((U_ref - U64).ravel()) / np.linalg.norm(U64.ravel())
+        say(f"  uninterrupted mixed vs FP64 after {k1 + k2} steps: rel L2 {form:.2e}")
+        assert 0 < rel <= form

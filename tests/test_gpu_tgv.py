@@ -19,25 +19,31 @@ def dgmod():
     return dg
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 1e-5)])
-def test_gradients_match_oracle(precision, tol):
-    """BR1 gradients of (u,v,w,T) on a curved, rotated mesh (N=5) against the oracle's gradUx/y/z rows 2..5:
-    relative L2 per direction within 1e-12 (FP64) / 1e-5 (mixed)."""
-    say(f"case: gradients {precision}, (3,2,2) N=5 curved rotated")
-    m = M.box_mesh((3, 2, 2), 5, lo=(-1, -1, -1), hi=(1, 1, 1), deform_amp=0.05, deform_phase=(0.1, 0.4, 0.2),
+@pytest.mark.parametrize("precision,tol,N,nvar", [("fp64", 1e-12, 5, 4), ("mixed", 1e-5, 5, 4), ("fp64", 1e-12, 5, 5),
+                                                  ("mixed", 1e-5, 5, 5), ("fp64", 1e-12, 11, 5)])
+def test_gradients_match_oracle(precision, tol, N, nvar):
+    """BR1 gradients on a curved, rotated mesh against the oracle's gradUx/y/z, which lifts the reference's default
+    variables (rho,u,v,w,T): nVarLifting 4 = its rows 2..5 (PP_OPTLIFT), 5 = all rows.  Relative L2 per direction
+    and variable within 1e-12 (FP64) / 1e-5 (mixed); N = 11 runs the high-order kernels."""
+    say(f"case: gradients {precision}, nVarLifting {nvar}, (3,2,2) N={N} curved rotated")
+    m = M.box_mesh((3, 2, 2), N, lo=(-1, -1, -1), hi=(1, 1, 1), deform_amp=0.05, deform_phase=(0.1, 0.4, 0.2),
                    rotations="random", seed=3)
     U = T.smooth_random(m.Elem_xGP, (-1, -1, -1), (1, 1, 1), seed=5)
-    op = dgmod().DGOperator(5, form="split", riemann="lf", precision=precision, kappa=1.4, R=1.0, Pr=0.72, mu0=1e-2)
+    op = dgmod().DGOperator(N, form="split", riemann="lf", precision=precision, kappa=1.4, R=1.0, Pr=0.72,
+                            mu0=1e-2)
     op.set_mesh(m)
     op.set_state(U)
-    g = op.get_gradients()
+    g = op.get_gradients(nvar)
     ref = O.Operator(m, split=True, riemann=O.RIEMANN_LF, kappa=1.4, R=1.0, Pr=0.72, mu0=1e-2)
     ref.rhs(U)
     n = m.N + 1
     for d, w in enumerate((3, 4, 5)):
-        r = ref.array(w).reshape((5, n, n, n, m.nElems), order="F")[1:5]
-        err = np.linalg.norm((g[d] - r).ravel()) / np.linalg.norm(r.ravel())
-        assert err <= tol, (d, err)
+        r = ref.array(w).reshape((5, n, n, n, m.nElems), order="F")[5 - nvar:]
+        for v in range(nvar):
+            err = np.linalg.norm((g[d][v] - r[v]).ravel()) / np.linalg.norm(r[v].ravel())
+            assert err <= tol, (d, v, err)
+    if nvar == 5:
+        assert np.array_equal(g[0][1:], op.get_gradients(4)[0])  # the same (u,v,w,T) rows either way
     op.close()
 
 
