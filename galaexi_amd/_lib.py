@@ -14,6 +14,7 @@ GALX_FORM_WEAK, GALX_FORM_SPLIT_PI = 0, 1
 GALX_RIEMANN_LF, GALX_RIEMANN_ROE, GALX_RIEMANN_ROEEF = 0, 1, 3
 GALX_PREC_FP64, GALX_PREC_MIXED = 0, 1
 GALX_ADJ_STAGES, GALX_ADJ_STEPS = 0, 1
+GALX_E_ARG = -1
 
 # every symbol declared in include/galx.h
 EXPORTS = (
@@ -24,7 +25,7 @@ EXPORTS = (
     "galx_rhs_vjp", "galx_rhs_vjp_device", "galx_lserk_forward_ckpt", "galx_lserk_adjoint",
     "galx_lserk_adjoint_device", "galx_lserk_stage", "galx_get_gradients", "galx_get_gradients_device",
     "galx_step_vjp", "galx_step_vjp_device", "galx_set_source", "galx_set_adjoint_storage",
-    "galx_get_lifted_gradients", "galx_get_lifted_gradients_device",
+    "galx_get_lifted_gradients", "galx_get_lifted_gradients_device", "galx_surf_int_noalloc",
 )
 
 
@@ -76,6 +77,7 @@ def lib():
         "galx_calc_dt": ([vp, dp, ip], i),
         "galx_prolong_to_face": ([i, i, i, i, i, ip, ip, dp, dp, dp], i),
         "galx_surf_int": ([i, i, i, i, i, ip, ip, dp, dp, d, d, dp], i),
+        "galx_surf_int_noalloc": ([i, i, i, i, i, ip, ip, dp, dp, d, d, dp], i),
         "galx_timers_enable": ([vp, i], i),
         "galx_timer_read": ([vp, C.c_char_p, dp, C.POINTER(C.c_long)], i),
         "galx_timers_reset": ([vp], i),

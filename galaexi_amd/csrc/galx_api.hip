@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1847,5 +1848,69 @@ extern "C" int galx_surf_int(int device, int nVar, int N, int nElems, int nSides
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(Ut, dU.p, sv, hipMemcpyDeviceToHost));
+    return GALX_OK;
+}
+
+// ----------------------------------------------------------------------------------------------------------
+// galx_surf_int_noalloc: galx_surf_int staged through a file-scope device array instead of device allocations.
+// The host-side name `scratch` is a stand-in, not a device address: host code uses only the address that
+// hipGetSymbolAddress returns for the current device, kept per device in g_scratch.
+// ----------------------------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[GALX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch;  // device ordinal -> device address of that device's scratch
+
+int scratch_address(int device, unsigned char** out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it != g_scratch.end()) {
+        *out = it->second;
+        return GALX_OK;
+    }
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(GALX_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch[device] = (unsigned char*)p;
+    *out = (unsigned char*)p;
+    return GALX_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
+
+extern "C" int galx_surf_int_noalloc(int device, int nVar, int N, int nElems, int nSides, const int* S2E,
+                                     const int* S2V2, const double* Fm, const double* Fs, double LHm, double LHp,
+                                     double* Ut)
+{
+    if (nVar < 1 || nVar > 16 || N < 1 || N > 30 || nElems < 1 || nSides < 1)
+        return fail(GALX_E_ARG, "bad arguments");
+    const int n = N + 1, n2 = n * n;
+    const size_t bE = 18UL * nElems * 4, bI = 30UL * n2 * 4, bF = (size_t)nVar * n2 * nSides * 8,
+                 bU = (size_t)nVar * n2 * n * nElems * 8;
+    const size_t oI = align256(bE), oM = oI + align256(bI), oS = oM + align256(bF), oU = oS + align256(bF);
+    const size_t need = oU + bU;
+    if (need > (size_t)GALX_NOALLOC_SCRATCH_BYTES)
+        return fail(GALX_E_ARG, "galx_surf_int_noalloc: %zu bytes of staging exceed its %u-byte scratch", need,
+                    (unsigned)GALX_NOALLOC_SCRATCH_BYTES);
+    std::vector<int> E2S, inv;
+    int rc = surf_int_prepare(nVar, N, nElems, nSides, S2E, S2V2, Fm, Fs, Ut, E2S, inv);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(device));
+    unsigned char* base = nullptr;
+    if ((rc = scratch_address(device, &base))) return rc;
+    HIPCHK(hipMemcpy(base, E2S.data(), bE, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(base + oI, inv.data(), bI, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(base + oM, Fm, bF, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(base + oS, Fs, bF, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(base + oU, Ut, bU, hipMemcpyHostToDevice));
+    launch_unit_surfint(nVar, N, nElems, (const int*)base, (const int*)(base + oI), (const double*)(base + oM),
+                        (const double*)(base + oS), LHm, LHp, (double*)(base + oU));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(Ut, base + oU, bU, hipMemcpyDeviceToHost));
     return GALX_OK;
 }

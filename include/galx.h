@@ -136,6 +136,19 @@ int galx_surf_int(int device, int nVar, int N, int nElems, int nSides, const int
                   const double *Flux_master, const double *Flux_slave, double L_HatMinus0, double L_HatPlusN,
                   double *Ut);
 
+/* galx_surf_int_noalloc: galx_surf_int with no device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The inputs and outputs are staged through a fixed device array of
+ * GALX_NOALLOC_SCRATCH_BYTES inside the library (held on every GPU the library runs on until the process ends)
+ * instead of per-call hipMalloc/hipFree; the call allocates no device memory and calls no allocating routine.
+ * Requests whose staging (ElemToSide, S2V2_inv, both flux arrays and Ut, each rounded up to 256 bytes) exceeds
+ * that size return GALX_E_ARG.  All calls on one GPU share its scratch: calls that overlap in time on the same GPU
+ * (from several host threads) race and must be serialised by the caller. */
+#define GALX_NOALLOC_SCRATCH_BYTES (16u << 20)
+int galx_surf_int_noalloc(int device, int nVar, int N, int nElems, int nSides, const int *SideToElem,
+                          const int *S2V2, const double *Flux_master, const double *Flux_slave,
+                          double L_HatMinus0, double L_HatPlusN, double *Ut);
+
 /* Discrete adjoint (form SPLIT_PI, riemann LF; GALX_E_ARG otherwise).  The VJPs are FP64 in both precisions;
  * a mixed-precision context records and recomputes its trajectory in the mixed form and accumulates the adjoint
  * in FP64 along it.  The reference has no

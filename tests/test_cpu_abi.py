@@ -159,3 +159,15 @@ def test_set_mesh_checks_degree_and_shapes():
     m.sJ = m.sJ[:, :, :, :7]
     with pytest.raises(ValueError, match="sJ"):
         _detached_operator(3).set_mesh(m)
+
+
+def test_surf_int_noalloc_refuses_oversize_before_touching_the_device():
+    """galx_surf_int_noalloc checks its staging size against its fixed scratch before any HIP call, so the
+    refusal is testable without a GPU: GALX_E_ARG and a message naming the scratch."""
+    L = _lib.lib()
+    N, nElems, nSides = 7, 4096, 12288
+    dummy_i = (C.c_int * 1)()
+    dummy_d = (C.c_double * 1)()
+    rc = L.galx_surf_int_noalloc(0, 5, N, nElems, nSides, dummy_i, dummy_i, dummy_d, dummy_d, 1.0, 1.0, dummy_d)
+    assert rc == _lib.GALX_E_ARG
+    assert b"scratch" in L.galx_last_error()
