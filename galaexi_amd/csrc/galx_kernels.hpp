@@ -19,8 +19,6 @@
 // 0.5*(F_v(master) + F_v(slave)).n (riemann.f90:638-700) needs only the normal viscous flux of each side, so the
 // sides exchange those 4 values per face node instead of the 12 gradient components.
 #pragma once
-#include <type_traits>
-
 #include "galx_physics.hpp"
 
 #ifndef GALX_UNROLL_L
@@ -31,11 +29,6 @@
 #endif
 #ifndef GALX_MIXED_WAVES
 #define GALX_MIXED_WAVES 6
-#endif
-// FP64 split form: the partner values of the flux differencing (rho,u,v,w,p,H and the direction's metric, all
-// halved) in LDS as five planes of double pairs, read with 16-byte loads, instead of nine planes of doubles
-#ifndef GALX_PAIR_PLANES
-#define GALX_PAIR_PLANES 1
 #endif
 #ifndef GALX_OWN_SLOT
 #define GALX_OWN_SLOT 1
@@ -706,13 +699,10 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     // mixed split form: the nine metric terms of every node stay in LDS (FP32, halves) in s_geo for the whole
     // kernel, so the volume terms read them there instead of from memory a second time (7.8 % faster at N=7)
     constexpr bool geo_lds = SPLIT && sizeof(T) == 4;
-    // FP64 split form: the partner data as pair planes (rho,u) (v,w) (p,H) (m0,m1) (m2,-), one 16-byte LDS read
-    // each, instead of nine 8-byte reads
-    constexpr bool pairs = SPLIT && sizeof(T) == 8 && GALX_PAIR_PLANES;
     // mixed form: the face tasks read the own trace record from the trace buffer's own slot (written from the same
     // state by the previous stage or by k_trace, so bitwise the record U would give)
     constexpr bool own_slot = sizeof(TT) == 4 && GALX_OWN_SLOT;
-    constexpr int w_c = SPLIT ? (geo_lds ? 10 : (pairs ? 14 : 13)) * n3p : 5 * n3p;  // volume fluxes / partners
+    constexpr int w_c = SPLIT ? (geo_lds ? 10 : 13) * n3p : 5 * n3p;  // volume fluxes / split-form partner data
     constexpr int w_d = 6 * 5 * n2;  // face results, new traces
     // split form: the i- and j-lines lie inside one 64-lane wave (n and n^2 divide 64: N = 1, 3, 7), so the pairs of
     // those lines are shared between lanes instead of formed twice
@@ -801,34 +791,13 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         // (rho,u,v,w,p,H) and metric, and the sum is doubled once at the end (exact)
         T* s_v = s_w;                                // [6][n3p] rho,u,v,w,p,H (halves)
         T* s_m = s_w + 6 * n3p;                      // [3][n3p] metric of the current direction (halves)
-        T* s_f = s_w + (geo_lds ? 6 : (pairs ? 10 : 9)) * n3p;  // [4][n3p] viscous contravariant flux (momentum,
-                                                                // energy) of the current direction
-        // pairs: [5][n3p] pair planes in place of s_v and s_m
-        using T2 = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
-        T2* s_p = reinterpret_cast<T2*>(s_w);
-        // the halved state and metric of partner node o
-        auto partner = [&](const T* sv, const T* sm, const T2* sp, int o, T* Po, T* Mo) {
-            if (pairs) {
-                const T2 a = sp[o], b = sp[n3p + o], c = sp[2 * n3p + o], dm = sp[3 * n3p + o], f = sp[4 * n3p + o];
-                Po[0] = a.x; Po[1] = a.y; Po[2] = b.x; Po[3] = b.y; Po[4] = c.x; Po[5] = c.y;
-                Mo[0] = dm.x; Mo[1] = dm.y; Mo[2] = f.x;
-            } else {
-#pragma unroll
-                for (int q = 0; q < 6; q++) Po[q] = sv[q * n3p + o];
-#pragma unroll
-                for (int c = 0; c < 3; c++) Mo[c] = sm[c * n3p + o];
-            }
-        };
+        T* s_f = s_w + (geo_lds ? 6 : 9) * n3p;      // [4][n3p] viscous contravariant flux (momentum, energy) of
+                                                     // the current direction
         const T hv[6] = {T(0.5) * pr[0], T(0.5) * pr[1], T(0.5) * pr[2], T(0.5) * pr[3], T(0.5) * pr[4],
                          T(0.5) * Hme};
         if (active) {
-            if (pairs) {
 #pragma unroll
-                for (int q = 0; q < 3; q++) s_p[q * n3p + pt] = T2{hv[2 * q], hv[2 * q + 1]};
-            } else {
-#pragma unroll
-                for (int q = 0; q < 6; q++) s_v[q * n3p + pt] = hv[q];
-            }
+            for (int q = 0; q < 6; q++) s_v[q * n3p + pt] = hv[q];
         }
         T tmp[5] = {0, 0, 0, 0, 0};
 #pragma unroll 1
@@ -848,11 +817,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                         // the prefetched values, kept in registers since the load (geo_reg)
                         mme[c] = d == 0 ? gm[c] : (d == 1 ? gm[3 + c] : gm[6 + c]);
                         hm[c] = T(0.5) * mme[c];
-                        if (!pairs) s_m[c * n3p + pt] = hm[c];
-                    }
-                    if (pairs) {
-                        s_p[3 * n3p + pt] = T2{hm[0], hm[1]};
-                        s_p[4 * n3p + pt] = T2{hm[2], T(0)};
+                        s_m[c * n3p + pt] = hm[c];
                     }
                 }
                 T f[5];
@@ -886,11 +851,11 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                     for (int s = 1; s <= n / 2; s++) {
                         const int b = (idx + s) & msk;          // partner formed here
                         const int o = ln(b);
-                        T Mo[3], Po[6];
-                        partner(s_v, s_m, s_p, o, Po, Mo);
+                        T Mo[3] = {s_m[o], s_m[n3p + o], s_m[2 * n3p + o]};
                         T G[5];
-                        split_volume_flux_pi_half<T>(hv[0], hv[1], hv[2], hv[3], hv[4], hv[5], Po[0], Po[1], Po[2],
-                                                     Po[3], Po[4], Po[5], hm, Mo, G);
+                        split_volume_flux_pi_half<T>(hv[0], hv[1], hv[2], hv[3], hv[4], hv[5], s_v[o],
+                                                     s_v[n3p + o], s_v[2 * n3p + o], s_v[3 * n3p + o],
+                                                     s_v[4 * n3p + o], s_v[5 * n3p + o], hm, Mo, G);
                         const T db = dp[s * n];
                         if (2 * s < n) {
                             const int src = lane0 + ((idx - s) & msk) * st;  // the lane that formed (a-s, a)
@@ -913,11 +878,11 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll GALX_UNROLL_SPLIT
                 for (int l = 0; l <= N; l++) {
                     const int o = ln(l);
-                    T Mo[3], Po[6];
-                    partner(s_v, s_m, s_p, o, Po, Mo);
+                    T Mo[3] = {s_m[o], s_m[n3p + o], s_m[2 * n3p + o]};
                     T G[5];
-                    split_volume_flux_pi_half<T>(hv[0], hv[1], hv[2], hv[3], hv[4], hv[5], Po[0], Po[1], Po[2],
-                                                 Po[3], Po[4], Po[5], hm, Mo, G);
+                    split_volume_flux_pi_half<T>(hv[0], hv[1], hv[2], hv[3], hv[4], hv[5], s_v[o], s_v[n3p + o],
+                                                 s_v[2 * n3p + o], s_v[3 * n3p + o], s_v[4 * n3p + o],
+                                                 s_v[5 * n3p + o], hm, Mo, G);
                     T dv = M.dvs(l, idx);
 #pragma unroll
                     for (int v = 0; v < 5; v++) tmp[v] = tmp[v] + dv * G[v];
