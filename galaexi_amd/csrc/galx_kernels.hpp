@@ -30,9 +30,6 @@
 #ifndef GALX_MIXED_WAVES
 #define GALX_MIXED_WAVES 6
 #endif
-#ifndef GALX_OWN_SLOT
-#define GALX_OWN_SLOT 1
-#endif
 
 namespace galx {
 
@@ -699,9 +696,6 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     // mixed split form: the nine metric terms of every node stay in LDS (FP32, halves) in s_geo for the whole
     // kernel, so the volume terms read them there instead of from memory a second time (7.8 % faster at N=7)
     constexpr bool geo_lds = SPLIT && sizeof(T) == 4;
-    // mixed form: the face tasks read the own trace record from the trace buffer's own slot (written from the same
-    // state by the previous stage or by k_trace, so bitwise the record U would give)
-    constexpr bool own_slot = sizeof(TT) == 4 && GALX_OWN_SLOT;
     constexpr int w_c = SPLIT ? (geo_lds ? 10 : 13) * n3p : 5 * n3p;  // volume fluxes / split-form partner data
     constexpr int w_d = 6 * 5 * n2;  // face results, new traces
     // split form: the i- and j-lines lie inside one 64-lane wave (n and n^2 divide 64: N = 1, 3, 7), so the pairs of
@@ -906,24 +900,18 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
         T Uo[5], fvo[4], Un[5], fvn[4];
         {
-            // FP64: from U at the face node, not from the own slot of the trace buffer: the same record, but that
-            // read goes to memory (measured 1.5 % slower in FP64 on MI355X).  Formed by the node threads from the
-            // state they hold and passed through LDS instead, it measured 6 % slower in FP64 at N=7: the state
-            // then stays live through the volume terms and spills.  Mixed form: the own slot, whose FP32 record is
-            // half the bytes of the FP64 state at the node (GALX_OWN_SLOT)
-            if (own_slot) {
-                const TT* to = utr + ((side * 2 + (flip ? 1 : 0)) * 5) * n2 + pq;
+            // from U at the face node, not from the own slot of the trace buffer: the same record, but that read
+            // goes to memory (measured slower on MI355X at N=7: 1.5 % in FP64, and 0.5 % in the mixed form, whose
+            // FP32 record is half the bytes of the FP64 state).  Formed by the node threads from the state they
+            // hold and passed through LDS instead, it measured 6 % slower in FP64 at N=7: the state then stays
+            // live through the volume terms and spills.
+            const double* uo = U + (long)e * 5 * n3 + node;
+            double c[5], tr[5];
 #pragma unroll
-                for (int v = 0; v < 5; v++) Uo[v] = T(to[v * n2]);
-            } else {
-                const double* uo = U + (long)e * 5 * n3 + node;
-                double c[5], tr[5];
+            for (int v = 0; v < 5; v++) c[v] = uo[v * n3];
+            trace_record<sizeof(TT) == 4>(c, gd.kappa, gd.pref, tr);
 #pragma unroll
-                for (int v = 0; v < 5; v++) c[v] = uo[v * n3];
-                trace_record<sizeof(TT) == 4>(c, gd.kappa, gd.pref, tr);
-#pragma unroll
-                for (int v = 0; v < 5; v++) Uo[v] = T(TT(tr[v]));  // exactly the record the neighbour reads
-            }
+            for (int v = 0; v < 5; v++) Uo[v] = T(TT(tr[v]));  // exactly the record the neighbour reads
         }
         const TT* vo = vtr + ((side * 2 + (flip ? 1 : 0)) * 4) * n2 + pq;  // written by k_lift of this element
 #pragma unroll

@@ -198,13 +198,15 @@ def test_calc_dt_detects_bad_density():
     assert err == 1
 
 
-@pytest.mark.parametrize("N", [3, 7])
-def test_rhs_mixed_vs_oracle(N):
+@pytest.mark.parametrize("N,form,riemann", [(3, "split", "lf"), (7, "split", "lf"), (3, "weak", "roe"),
+                                            (5, "split", "roeentropyfix"), (5, "weak", "lf"), (7, "weak", "roe"),
+                                            (7, "split", "roe")])
+def test_rhs_mixed_vs_oracle(N, form, riemann):
     """Mixed form (FP32 fluxes/integrals/lifting/traces, FP64 U, metrics, sJ): Ut within 1e-4 relative L2
-    (FP32 round-off amplified by the cancellation in the flux divergence)."""
-    say(f"case: rhs mixed N={N}")
+    (FP32 round-off amplified by the cancellation in the flux divergence), every form and solver it is built for."""
+    say(f"case: rhs mixed N={N} {form} {riemann}")
     m, U = make_case((2, 2, 2), N, "curvrot", seed=2)
-    Ut, Ut_ref = run_pair(m, U, "split", "lf", precision="mixed")
+    Ut, Ut_ref = run_pair(m, U, form, riemann, precision="mixed")
     for v in range(5):
         assert rel_l2(Ut[v], Ut_ref[v]) <= 1e-4, (v, rel_l2(Ut[v], Ut_ref[v]))
 
