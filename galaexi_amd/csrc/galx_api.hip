@@ -528,9 +528,9 @@ static Mesh mesh_of(galx_ctx* c)
 {
     return Mesh{c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dSgeo, c->dGeo32, c->dFgeo32, c->dSgeo32, c->dWs};
 }
-static GasD gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, c->pref}; }
+static GasD gas_of(galx_ctx* c) { return make_gasd(c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, c->pref); }
 // the adjoint kernels and their FP64 trace records use the pressure itself (no offset)
-static GasD adj_gas_of(galx_ctx* c) { return GasD{c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, 0.0}; }
+static GasD adj_gas_of(galx_ctx* c) { return make_gasd(c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, 0.0); }
 
 // two pinned host buffers that alternate as staging for host-to-device copies: next() returns a buffer whose
 // previous copy has finished, issued() marks the copies just enqueued from it

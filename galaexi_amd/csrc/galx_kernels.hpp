@@ -99,15 +99,26 @@ __device__ __forceinline__ bool on_face(int ls, int i, int j, int k, int& ab)
     }
 }
 
-// gas constants and the pressure offset in FP64; each kernel derives its Gas<T>
+// gas constants and the pressure offset in FP64, with the derived constants the kernels use (make_gasd fills
+// them); each kernel derives its Gas<T>
 struct GasD {
     double kappa, R, mu, lambda, pref;
+    double rR;      // 1/R
+    double rkm1;    // 1/(kappa-1)
+    double eshift;  // pref/(kappa-1): the energy offset of a state that carries p' = p - pref
+    double hshift;  // kappa*pref/(kappa-1): the enthalpy offset
 };
+
+__host__ __device__ inline GasD make_gasd(double kappa, double R, double mu, double lambda, double pref)
+{
+    return GasD{kappa, R, mu, lambda, pref, 1.0 / R, 1.0 / (kappa - 1.0), pref / (kappa - 1.0),
+                kappa * pref / (kappa - 1.0)};
+}
 
 template <typename T>
 __device__ __forceinline__ Gas<T> make_gas(const GasD& d)
 {
-    return Gas<T>{T(d.kappa), T(d.R), T(d.mu), T(d.lambda), T(d.pref), T(d.kappa * d.pref / (d.kappa - 1.0))};
+    return Gas<T>{T(d.kappa), T(d.R), T(d.mu), T(d.lambda), T(d.pref), T(d.hshift), T(d.rR), T(d.rkm1)};
 }
 
 struct Mesh {
@@ -394,23 +405,22 @@ __device__ __forceinline__ void load_state(int e, int tid, const double* __restr
     double c[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) c[v] = u[v * n3];
-    // the mixed form rounds these to FP32: its 1/rho comes from the refined hardware reciprocal
-    constexpr bool fast = sizeof(T) == 4;
-    double sRho = fast ? rcp64_fast(c[0]) : 1.0 / c[0];
+    // the mixed form rounds these to FP32: its 1/rho takes one Newton step fewer
+    double sRho = sizeof(T) == 4 ? rcp64_fast(c[0]) : rcp64(c[0]);
     double vx = c[1] * sRho, vy = c[2] * sRho, vz = c[3] * sRho;
     double p = (gd.kappa - 1.0) * (c[4] - 0.5 * (c[1] * vx + c[2] * vy + c[3] * vz));
     uc[0] = T(c[0]);
     uc[1] = T(c[1]);
     uc[2] = T(c[2]);
     uc[3] = T(c[3]);
-    uc[4] = T(c[4] - gd.pref / (gd.kappa - 1.0));
+    uc[4] = T(c[4] - gd.eshift);
     pr[0] = T(c[0]);
     pr[1] = T(vx);
     pr[2] = T(vy);
     pr[3] = T(vz);
     pr[4] = T(p - gd.pref);
-    pr[5] = T(p * sRho / gd.R);
-    H = fast ? T((c[4] + p) * sRho) : T((c[4] + p) / c[0]);
+    pr[5] = T(p * sRho * gd.rR);
+    H = T((c[4] + p) * sRho);
 }
 
 template <typename TT>
@@ -656,21 +666,20 @@ __device__ __forceinline__ void face_flux(const T* tro, const T* fvo, const T* t
     trace_state(tro, gas, Uo, Po);
     trace_state(trn, gas, Un, Pn);
     T nv[3] = {fg[0], fg[n2], fg[2 * n2]};
-    const T *UL = flip ? Un : Uo, *UR = flip ? Uo : Un, *PL = flip ? Pn : Po, *PR = flip ? Po : Pn;
     if (SPLIT && RIEMANN == RIEMANN_LF) {
-        lf_pi_cartesian<T>(UL, UR, PL, PR, nv, gas, F);
+        // own side first; the master is the own side unless flipped
+        lf_pi_cartesian<T>(Uo, Un, Po, Pn, nv, flip ? T(-1) : T(1), gas, F);
     } else {
+        const T *UL = flip ? Un : Uo, *UR = flip ? Uo : Un, *PL = flip ? Pn : Po, *PR = flip ? Po : Pn;
         T t1[3] = {fg[3 * n2], fg[4 * n2], fg[5 * n2]};
         T t2[3] = {fg[6 * n2], fg[7 * n2], fg[8 * n2]};
         riemann_point<T, SPLIT>(RIEMANN, UL, UR, PL, PR, nv, t1, t2, gas, F);
     }
-    // ViscousFlux (riemann.f90:638-700): 0.5*(F_v(master) + F_v(slave)).n
-    const T* fvm = flip ? fvn : fvo;
-    const T* fvs = flip ? fvo : fvn;
+    // ViscousFlux (riemann.f90:638-700): 0.5*(F_v(master) + F_v(slave)).n, summed in an order both sides share
     T se = fg[9 * n2];
     F[0] = F[0] * se;
 #pragma unroll
-    for (int v = 1; v < 5; v++) F[v] = ((F[v] + T(0.5) * fvm[v - 1]) + T(0.5) * fvs[v - 1]) * se;
+    for (int v = 1; v < 5; v++) F[v] = (F[v] + (T(0.5) * fvo[v - 1] + T(0.5) * fvn[v - 1])) * se;
 }
 
 // minimum waves per SIMD for k_rhs: 4 lets two 512-thread workgroups share a CU at N=7 in FP64 (registers capped

@@ -17,17 +17,32 @@ namespace galx {
 
 enum { RIEMANN_LF = 0, RIEMANN_ROE = 1, RIEMANN_ROEEF = 3 };
 
+// rR = 1/R and rkm1 = 1/(kappa-1), formed once on the host: the pointwise routines multiply by them where the
+// reference divides by R and by kappa-1
 template <typename T>
 struct Gas {
-    T kappa, R, mu, lambda, pref, hshift;
+    T kappa, R, mu, lambda, pref, hshift, rR, rkm1;
 };
 
-// square root, reciprocal and quotient: IEEE-exact in FP64; in FP32 (the mixed form) the hardware instructions
-// (v_sqrt_f32, v_rcp_f32, about 1 ulp), whose error is far below the rounding of the FP32 fluxes they feed, at one
-// instruction instead of a correction sequence (the mixed kernels are VALU-bound)
+// 1/x in FP64 from the hardware reciprocal (v_rcp_f64) refined by two Newton steps: within an ulp of the IEEE
+// quotient for the normal, finite densities it is taken of, in five instructions instead of the scaled division
+// sequence (about twelve, two of them long-latency)
+__device__ __forceinline__ double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// square root, reciprocal and quotient.  FP64: IEEE square root and quotient, the reciprocal of rcp64.  FP32 (the
+// mixed form): the hardware instructions (v_sqrt_f32, v_rcp_f32, about 1 ulp), whose error is far below the
+// rounding of the FP32 fluxes they feed, at one instruction instead of a correction sequence (the mixed kernels
+// are VALU-bound)
 __device__ __forceinline__ double gsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float gsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ double grcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ double grcp(double x) { return rcp64(x); }
 __device__ __forceinline__ float grcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double gdiv(double a, double b) { return a / b; }
 __device__ __forceinline__ float gdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
@@ -46,7 +61,7 @@ __device__ __forceinline__ void cons_to_prim(const T* c, T* p, const Gas<T>& g)
     p[2] = c[2] * sRho;
     p[3] = c[3] * sRho;
     p[4] = (g.kappa - T(1)) * (c[4] - T(0.5) * (c[1] * p[1] + c[2] * p[2] + c[3] * p[3]));
-    p[5] = gdiv(p[4] * sRho, g.R);
+    p[5] = p[4] * sRho * g.rR;
 }
 
 // face state from a trace record (rho, rho*u, rho*v, rho*w, p'): Uc = (rho, rho*u, rho*v, rho*w, E'),
@@ -60,16 +75,16 @@ __device__ __forceinline__ void trace_state(const T* tr, const Gas<T>& g, T* Uc,
     P[2] = tr[2] * sRho;
     P[3] = tr[3] * sRho;
     P[4] = tr[4];
-    P[5] = gdiv((tr[4] + g.pref) * sRho, g.R);
+    P[5] = (tr[4] + g.pref) * sRho * g.rR;
     Uc[0] = tr[0];
     Uc[1] = tr[1];
     Uc[2] = tr[2];
     Uc[3] = tr[3];
-    Uc[4] = gdiv(tr[4], g.kappa - T(1)) + T(0.5) * (tr[1] * P[1] + tr[2] * P[2] + tr[3] * P[3]);
+    Uc[4] = tr[4] * g.rkm1 + T(0.5) * (tr[1] * P[1] + tr[2] * P[2] + tr[3] * P[3]);
 }
 
-// 1/x in FP64 from the hardware reciprocal (v_rcp_f64) refined by one Newton step: for FP64 values that are rounded
-// to FP32 afterwards (the mixed form's trace records and node states), three instructions instead of a division
+// the same with one Newton step: for FP64 values that are rounded to FP32 afterwards (the mixed form's trace
+// records and node states), three instructions
 __device__ __forceinline__ double rcp64_fast(double x)
 {
     double r = __builtin_amdgcn_rcp(x);
@@ -81,7 +96,7 @@ __device__ __forceinline__ double rcp64_fast(double x)
 template <bool FAST = false>
 __device__ __forceinline__ void trace_record(const double* c, double kappa, double pref, double* tr)
 {
-    double sRho = FAST ? rcp64_fast(c[0]) : 1.0 / c[0];
+    double sRho = FAST ? rcp64_fast(c[0]) : rcp64(c[0]);
     double u = c[1] * sRho, v = c[2] * sRho, w = c[3] * sRho;
     double p = (kappa - 1.0) * (c[4] - 0.5 * (c[1] * u + c[2] * v + c[3] * w));
     tr[0] = c[0];
@@ -154,8 +169,9 @@ __device__ __forceinline__ void riemann_solver(int solver, const Ext<T>& L, cons
 {
     const T kappa = g.kappa, hshift = g.hshift;
     if (solver == RIEMANN_LF) {
-        T cL = gsqrt(kappa * (L.pres + g.pref) * L.srho), cR = gsqrt(kappa * (R.pres + g.pref) * R.srho);
-        T lmax = gmax(gabs(R.vel[0]), gabs(L.vel[0])) + gmax(cL, cR);
+        // max(c_L, c_R) as the root of the larger c^2: the square root is monotone, so the same value, one root
+        T lmax = gmax(gabs(R.vel[0]), gabs(L.vel[0])) +
+                 gsqrt(gmax(kappa * (L.pres + g.pref) * L.srho, kappa * (R.pres + g.pref) * R.srho));
         if (!SPLIT) {
 #pragma unroll
             for (int v = 0; v < 5; v++) F[v] = T(0.5) * ((FL[v] + FR[v]) - lmax * (ext_cons(R, v) - ext_cons(L, v)));
@@ -251,24 +267,29 @@ __device__ __forceinline__ void riemann_point(int solver, const T* UL, const T* 
 // Cartesian flux F.n, without the normal-frame rotation of the Riemann routine (riemann.f90:216-289): rotating into
 // (n,t1,t2) and back leaves the PI momentum flux 1/8 {rho} {u_n} {u} + 1/2 {p} n with {x} = x_L + x_R and the
 // dissipation -lmax/2 (U_R - U_L) unchanged, so only u.n is needed (algebraically identical to riemann_point).
+// The states come in the caller's order a, b with (L, R) = (a, b) for sgn = +1 and (b, a) for sgn = -1: every
+// operation of the central part and of lmax is symmetric in L and R (sums, products, max), and
+// U_R - U_L = sgn (U_b - U_a) exactly, so the flux is the one of the (L, R) order bit for bit and the two elements
+// of a face form the same value without selecting their operands lane by lane.
 template <typename T>
-__device__ __forceinline__ void lf_pi_cartesian(const T* UL, const T* UR, const T* PL, const T* PR, const T* nv,
+__device__ __forceinline__ void lf_pi_cartesian(const T* Ua, const T* Ub, const T* Pa, const T* Pb, const T* nv, T sgn,
                                                 const Gas<T>& g, T* F)
 {
-    const T vnL = PL[1] * nv[0] + PL[2] * nv[1] + PL[3] * nv[2];
-    const T vnR = PR[1] * nv[0] + PR[2] * nv[1] + PR[3] * nv[2];
-    const T sRhoL = grcp(UL[0]), sRhoR = grcp(UR[0]);
-    const T HL = (UL[4] + PL[4] + g.hshift) * sRhoL, HR = (UR[4] + PR[4] + g.hshift) * sRhoR;
-    const T cL = gsqrt(g.kappa * (PL[4] + g.pref) * sRhoL), cR = gsqrt(g.kappa * (PR[4] + g.pref) * sRhoR);
-    const T lmax = gmax(gabs(vnR), gabs(vnL)) + gmax(cL, cR);
-    const T sr = UL[0] + UR[0], su = vnL + vnR, q = T(0.125) * sr * su, sp = T(0.5) * (PL[4] + PR[4]);
+    const T vna = Pa[1] * nv[0] + Pa[2] * nv[1] + Pa[3] * nv[2];
+    const T vnb = Pb[1] * nv[0] + Pb[2] * nv[1] + Pb[3] * nv[2];
+    const T sRhoA = grcp(Ua[0]), sRhoB = grcp(Ub[0]);
+    const T Ha = (Ua[4] + Pa[4] + g.hshift) * sRhoA, Hb = (Ub[4] + Pb[4] + g.hshift) * sRhoB;
+    // max(c_L, c_R) = sqrt(max(c_L^2, c_R^2)) (monotone root): one square root instead of two
+    const T lmax = gmax(gabs(vna), gabs(vnb)) +
+                   gsqrt(gmax(g.kappa * (Pa[4] + g.pref) * sRhoA, g.kappa * (Pb[4] + g.pref) * sRhoB));
+    const T sr = Ua[0] + Ub[0], su = vna + vnb, q = T(0.125) * sr * su, sp = T(0.5) * (Pa[4] + Pb[4]);
     F[0] = T(2) * q;
 #pragma unroll
-    for (int d = 0; d < 3; d++) F[1 + d] = q * (PL[1 + d] + PR[1 + d]) + sp * nv[d];
-    F[4] = q * (HL + HR);
-    const T hl = T(0.5) * lmax;
+    for (int d = 0; d < 3; d++) F[1 + d] = q * (Pa[1 + d] + Pb[1 + d]) + sp * nv[d];
+    F[4] = q * (Ha + Hb);
+    const T hl = T(0.5) * lmax * sgn;
 #pragma unroll
-    for (int v = 0; v < 5; v++) F[v] = F[v] - hl * (UR[v] - UL[v]);
+    for (int v = 0; v < 5; v++) F[v] = F[v] - hl * (Ub[v] - Ua[v]);
 }
 
 // gradients of the lifted variables in the order used here: g[4*dir + c], dir in {x,y,z}, c in {u,v,w,T}
