@@ -216,43 +216,85 @@ __device__ __forceinline__ void load_mats(Mats<N, T, TR>& M, const T* __restrict
 //   s_nb    : T[6][5][n2] (optional) receives the neighbour trace records of the six faces, for later face tasks
 // returns g[12] = grad(u,v,w,T) in x, then y, then z for node tid (valid if tid < n3).
 // ---------------------------------------------------------------------------------------------------------
-// BR1 lifting flux 0.5*SurfElem*(-L_master + L_slave) and the normal of every face node of element e
-// (the face-task half of lift_gradient); s_lflux [6][7][n2], s_nb as in lift_gradient
+// one face task: face node ab of local side ls of element e, with its side, flip and master (p,q) index (from the
+// connectivity and the inverse S2V2 table: two dependent loads)
+struct FaceTask {
+    long side;
+    int ls, ab, flip, pq;
+};
+
+template <int N>
+__device__ __forceinline__ FaceTask face_task(int e, int t, const Mesh& mesh)
+{
+    constexpr int n2 = (N + 1) * (N + 1);
+    FaceTask f;
+    f.ls = t / n2 + 1;
+    f.ab = t - (f.ls - 1) * n2;
+    const int c = mesh.conn[e * 6 + f.ls - 1];
+    f.side = c >> 3;
+    f.flip = c & 7;
+    f.pq = mesh.inv[(f.flip * 6 + f.ls - 1) * n2 + f.ab];
+    return f;
+}
+
+// the memory operands of a lifting face task: the neighbour's trace record, the normal and SurfElem
+template <typename T>
+struct LiftPre {
+    T tr[5], nv[3], se;
+};
+
+template <int N, typename T, typename TT>
+__device__ __forceinline__ void lift_prefetch(const FaceTask& f, const Mesh& mesh, const TT* __restrict__ utr,
+                                              LiftPre<T>& p)
+{
+    constexpr int n2 = (N + 1) * (N + 1);
+    const TT* nb = utr + ((f.side * 2 + (f.flip ? 0 : 1)) * 5) * n2 + f.pq;
+#pragma unroll
+    for (int v = 0; v < 5; v++) p.tr[v] = T(nb[v * n2]);
+    const T* fg = GeoOf<T>::fgeo(mesh) + f.side * 10 * n2 + f.pq;
+    p.nv[0] = fg[0];
+    p.nv[1] = fg[n2];
+    p.nv[2] = fg[2 * n2];
+    p.se = fg[9 * n2];
+}
+
+// BR1 lifting flux 0.5*SurfElem*(-L_master + L_slave) and the normal of one face node (s_lflux [6][7][n2]); the
+// neighbour's trace record also to s_nb [6][5][n2] when given
+template <int N, typename T>
+__device__ __forceinline__ void lift_face_task(const FaceTask& f, const LiftPre<T>& p, const T* s_lift, T* s_lflux,
+                                               const Gas<T>& gas, T* s_nb)
+{
+    constexpr int n = N + 1, n2 = n * n;
+    const int node = face_vol_node<N>(f.ls, f.ab % n, f.ab / n);
+    if (s_nb) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) s_nb[(f.ls - 1) * 5 * n2 + v * n2 + f.ab] = p.tr[v];
+    }
+    T uc[5], pr[6];
+    trace_state(p.tr, gas, uc, pr);
+    T Lnb[4] = {pr[1], pr[2], pr[3], pr[5]};
+    T* o = s_lflux + (f.ls - 1) * 7 * n2 + f.ab;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        T own = s_lift[q * Lay<N>::n3 + Lay<N>::at(node)];
+        T Lm = f.flip ? Lnb[q] : own, Lsl = f.flip ? own : Lnb[q];
+        o[q * n2] = T(0.5) * p.se * (T(-1) * Lm + Lsl);
+    }
+    o[4 * n2] = p.nv[0];
+    o[5 * n2] = p.nv[1];
+    o[6 * n2] = p.nv[2];
+}
+
+// the lifting face tasks of every face node of element e
 template <int N, typename T, typename TT>
 __device__ __forceinline__ void lift_face_fluxes(int e, int tid, const Mesh& mesh, const TT* __restrict__ utr,
                                                  const T* s_lift, T* s_lflux, const Gas<T>& gas, T* s_nb)
 {
-    using D = Dm<N>;
-    constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
-    for (int t = tid; t < D::nf; t += D::block) {
-        int ls = t / n2 + 1, ab = t - (ls - 1) * n2, a = ab % n, b = ab / n;
-        int node = face_vol_node<N>(ls, a, b);
-        int c = mesh.conn[e * 6 + ls - 1];
-        long side = c >> 3;
-        int flip = c & 7;
-        int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
-        const TT* nb = utr + ((side * 2 + (flip ? 0 : 1)) * 5) * n2 + pq;
-        T tr[5], uc[5], pr[6];
-#pragma unroll
-        for (int v = 0; v < 5; v++) tr[v] = T(nb[v * n2]);
-        if (s_nb) {
-#pragma unroll
-            for (int v = 0; v < 5; v++) s_nb[(ls - 1) * 5 * n2 + v * n2 + ab] = tr[v];
-        }
-        trace_state(tr, gas, uc, pr);
-        T Lnb[4] = {pr[1], pr[2], pr[3], pr[5]};
-        const T* fg = GeoOf<T>::fgeo(mesh) + side * 10 * n2 + pq;
-        T se = fg[9 * n2];
-        T* o = s_lflux + (ls - 1) * 7 * n2 + ab;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            T own = s_lift[q * Lay<N>::n3 + Lay<N>::at(node)];
-            T Lm = flip ? Lnb[q] : own, Lsl = flip ? own : Lnb[q];
-            o[q * n2] = T(0.5) * se * (T(-1) * Lm + Lsl);
-        }
-        o[4 * n2] = fg[0];
-        o[5 * n2] = fg[n2];
-        o[6 * n2] = fg[2 * n2];
+    for (int t = tid; t < Dm<N>::nf; t += Dm<N>::block) {
+        const FaceTask f = face_task<N>(e, t, mesh);
+        LiftPre<T> p;
+        lift_prefetch<N, T, TT>(f, mesh, utr, p);
+        lift_face_task<N, T>(f, p, s_lift, s_lflux, gas, s_nb);
     }
 }
 
@@ -534,26 +576,24 @@ __global__ void __launch_bounds__(Dm<N>::block, (LiftWaves<N, T>::value))
         }
     }
     __syncthreads();
+    // F_v.n_master of each own face node (the index and normal loaded again: cache hits, and no registers held
+    // through the gradients)
     for (int t = tid; t < D::nf; t += D::block) {
-        int ls = t / n2 + 1, ab = t - (ls - 1) * n2, a = ab % n, b = ab / n;
-        int node = face_vol_node<N>(ls, a, b);
-        int c = mesh.conn[e * 6 + ls - 1];
-        long side = c >> 3;
-        int flip = c & 7;
-        int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
-        const T* fg = GeoOf<T>::fgeo(mesh) + side * 10 * n2 + pq;
-        T nv[3] = {fg[0], fg[n2], fg[2 * n2]};
+        const FaceTask f = face_task<N>(e, t, mesh);
+        const T* fg = GeoOf<T>::fgeo(mesh) + f.side * 10 * n2 + f.pq;
+        const T nv[3] = {fg[0], fg[n2], fg[2 * n2]};
+        const int node = face_vol_node<N>(f.ls, f.ab % n, f.ab / n);
         const int pn = Lay<N>::at(node);
         T P[4] = {T(0), s_lift[pn], s_lift[n3p + pn], s_lift[2 * n3p + pn]};
         T G[12];
-        const T* sg = s_g + (ls - 1) * 12 * n2 + ab;
+        const T* sg = s_g + (f.ls - 1) * 12 * n2 + f.ab;
 #pragma unroll
         for (int q = 0; q < 12; q++) G[q] = sg[q * n2];
-        T f[5];
-        normal_diff_flux(P, G, nv, gas, f);
-        TT* o = vtr + ((side * 2 + (flip ? 1 : 0)) * 4) * n2 + pq;
+        T fl[5];
+        normal_diff_flux(P, G, nv, gas, fl);
+        TT* o = vtr + ((f.side * 2 + (f.flip ? 1 : 0)) * 4) * n2 + f.pq;
 #pragma unroll
-        for (int q = 0; q < 4; q++) o[q * n2] = TT(f[1 + q]);
+        for (int q = 0; q < 4; q++) o[q * n2] = TT(fl[1 + q]);
     }
 }
 
