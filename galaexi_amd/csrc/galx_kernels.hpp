@@ -99,6 +99,35 @@ __device__ __forceinline__ bool on_face(int ls, int i, int j, int k, int& ab)
     }
 }
 
+// do the lines of direction d of an element lie inside one 64-lane wave (threads = nodes i + n (j + n k))?  i-lines
+// if n divides 64, j-lines if n^2 does, k-lines if the element fits one wave.  The LDS exchanges along such lines
+// need only the wave's own ordering, no workgroup barrier.
+template <int N>
+__device__ __forceinline__ bool lines_in_wave(int d)
+{
+    constexpr int n = N + 1;
+    return d == 0 ? 64 % n == 0 : (d == 1 ? 64 % (n * n) == 0 : n * n * n <= 64);
+}
+
+// orders the wave's LDS accesses before and after this point (a wave's LDS instructions execute in order; this
+// keeps the compiler from moving them across)
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the barrier of an exchange along the lines of direction d
+template <int N>
+__device__ __forceinline__ void line_sync(int d)
+{
+    if (lines_in_wave<N>(d))
+        wave_lds_sync();
+    else
+        __syncthreads();
+}
+
 // gas constants and the pressure offset in FP64, with the derived constants the kernels use (make_gasd fills
 // them); each kernel derives its Gas<T>
 struct GasD {
@@ -320,26 +349,15 @@ __device__ __forceinline__ void load_surf_geo(int e, int t, const Mesh& mesh, T*
     for (int q = 0; q < 10; q++) gm[q] = ge[q * ns];
 }
 
-// gm: the node's Ja^1(3), Ja^2(3), Ja^3(3), sJ (load_node_geo), loaded by the caller ahead of the barrier that
-// precedes this call so that their latency overlaps the face tasks
-template <int N, typename T, class MT>
-__device__ __forceinline__ void node_gradient(int node, const MT& M, const T* s_lift, const T* s_lflux,
-                                              T LHm, T LHp, const T* gm, T* g)
+// the BR1 gradient of volume node `node` from its reference-direction derivatives gxi, get, gze of (u,v,w,T), the
+// metrics gm (Ja^1(3), Ja^2(3), Ja^3(3), sJ) and the surface terms of the lifting fluxes (lifting_volint.t90:262-328,
+// surfint.t90:591-725)
+template <int N, typename T>
+__device__ __forceinline__ void gradient_combine(int node, const T* s_lflux, T LHm, T LHp, const T* gm, const T* gxi,
+                                                 const T* get, const T* gze, T* g)
 {
     constexpr int n = N + 1, n2 = n * n;
-    constexpr int np = Lay<N>::np, n3p = Lay<N>::n3;
     const int i = node % n, j = (node / n) % n, k = node / n2;
-    T gxi[4] = {0, 0, 0, 0}, get[4] = {0, 0, 0, 0}, gze[4] = {0, 0, 0, 0};
-#pragma unroll GALX_UNROLL_L
-    for (int l = 0; l <= N; l++) {
-        T di = M.dt(l, i), dj = M.dt(l, j), dk = M.dt(l, k);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            gxi[q] = gxi[q] + di * s_lift[q * n3p + Lay<N>::idx(l, j, k)];
-            get[q] = get[q] + dj * s_lift[q * n3p + Lay<N>::idx(i, l, k)];
-            gze[q] = gze[q] + dk * s_lift[q * n3p + Lay<N>::idx(i, j, l)];
-        }
-    }
     const T mf[3] = {gm[0], gm[1], gm[2]}, mg[3] = {gm[3], gm[4], gm[5]}, mh[3] = {gm[6], gm[7], gm[8]};
     const T sJ = gm[9];
     T loc[12];
@@ -369,6 +387,108 @@ __device__ __forceinline__ void node_gradient(int node, const MT& M, const T* s_
         g[4 + q] = sJ * (vy + loc[4 + q]);
         g[8 + q] = sJ * (vz + loc[8 + q]);
     }
+}
+
+// gm: the node's Ja^1(3), Ja^2(3), Ja^3(3), sJ (load_node_geo), loaded by the caller ahead of the barrier that
+// precedes this call so that their latency overlaps the face tasks
+template <int N, typename T, class MT>
+__device__ __forceinline__ void node_gradient(int node, const MT& M, const T* s_lift, const T* s_lflux,
+                                              T LHm, T LHp, const T* gm, T* g)
+{
+    constexpr int n = N + 1, n2 = n * n;
+    constexpr int n3p = Lay<N>::n3;
+    const int i = node % n, j = (node / n) % n, k = node / n2;
+    T gxi[4] = {0, 0, 0, 0}, get[4] = {0, 0, 0, 0}, gze[4] = {0, 0, 0, 0};
+#pragma unroll GALX_UNROLL_L
+    for (int l = 0; l <= N; l++) {
+        T di = M.dt(l, i), dj = M.dt(l, j), dk = M.dt(l, k);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            gxi[q] = gxi[q] + di * s_lift[q * n3p + Lay<N>::idx(l, j, k)];
+            get[q] = get[q] + dj * s_lift[q * n3p + Lay<N>::idx(i, l, k)];
+            gze[q] = gze[q] + dk * s_lift[q * n3p + Lay<N>::idx(i, j, l)];
+        }
+    }
+    gradient_combine<N, T>(node, s_lflux, LHm, LHp, gm, gxi, get, gze, g);
+}
+
+// 16x16x4 matrix-core step of type T: D = A B + C with A 16x4, B 4x16; lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16], and register r of lane l holds D[row(l / 16, r)][l % 16] (gfx950, checked on the device: rows
+// 4 (l / 16) + r in FP32, (l / 16) + 4 r in FP64)
+template <typename T>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    __device__ static __forceinline__ v4 step(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ int row(int kq, int r) { return kq + 4 * r; }
+};
+template <>
+struct Mfma16<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    __device__ static __forceinline__ v4 step(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ int row(int kq, int r) { return 4 * kq + r; }
+};
+
+// BR1 gradient of the thread's own node at N = 7 (512 threads, node = thread), the xi- and eta-line derivatives on
+// the matrix cores.  The 8 x 8 nodes of a k-plane are one wave, and so are their i- and j-lines: for each of the two
+// directions one chain of four 16x16x4 steps forms D F for the 8 lines x 4 variables of the wave's plane, with
+// A = blockdiag(D, D) (two column groups of 16 line-variables) and B the lines' values from LDS.  The results reach
+// the node lanes through the wave's own scratch (s_scr, 4 x 64 values; no workgroup barrier, the wave's LDS
+// operations are ordered).  The zeta-lines cross the waves and stay on the VALU, where D(k, l) is uniform in the
+// wave.
+template <typename T, class MT>
+__device__ __forceinline__ void node_gradient_mfma(int tid, const MT& M, const T* s_lift, const T* s_lflux, T* s_scr,
+                                                   T LHm, T LHp, const T* gm, T* g)
+{
+    constexpr int N = 7, n3p = Lay<N>::n3;
+    const int lane = tid & 63, w = tid >> 6;  // the wave's plane k = w
+    const int r16 = lane & 15, kq = lane >> 4;
+    T a[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int c = 4 * s + kq;  // column of blockdiag(D, D); D(a, l) = dt(l, a)
+        a[s] = ((r16 >> 3) == (c >> 3)) ? M.dt(c & 7, r16 & 7) : T(0);
+    }
+    T gd2[2][4];
+#pragma unroll
+    for (int dir = 0; dir < 2; dir++) {
+        typename Mfma16<T>::v4 acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int pos = 4 * (s & 1) + kq;           // line position of B's row
+            const int col = 16 * (s >> 1) + r16;        // line-variable: line t, variable q
+            const int t = col & 7, q = col >> 3;
+            const int node = dir == 0 ? pos + 8 * t + 64 * w : t + 8 * pos + 64 * w;
+            acc = Mfma16<T>::step(a[s], s_lift[q * n3p + node], acc);
+        }
+        T* scr = s_scr + w * 256;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = Mfma16<T>::row(kq, r), ap = row & 7;  // output line position
+            const int col = 16 * (row >> 3) + r16, t = col & 7, q = col >> 3;
+            scr[q * 64 + (dir == 0 ? ap + 8 * t : t + 8 * ap)] = acc[r];
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int q = 0; q < 4; q++) gd2[dir][q] = scr[q * 64 + lane];
+        wave_lds_sync();
+    }
+    const int i = tid & 7, j = (tid >> 3) & 7;
+    T gze[4] = {0, 0, 0, 0};
+#pragma unroll GALX_UNROLL_L
+    for (int l = 0; l <= N; l++) {
+        const T dk = M.dt(l, w);
+#pragma unroll
+        for (int q = 0; q < 4; q++) gze[q] = gze[q] + dk * s_lift[q * n3p + Lay<N>::idx(i, j, l)];
+    }
+    gradient_combine<N, T>(tid, s_lflux, LHm, LHp, gm, gd2[0], gd2[1], gze, g);
 }
 
 template <int N, typename T, bool TR>
@@ -741,7 +861,12 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     static_assert(D::node_per_thread, "one volume node per thread: N <= 9");
     constexpr int n = D::n, n2 = D::n2, n3 = D::n3;
     constexpr int np = Lay<N>::np, n3p = Lay<N>::n3;  // padded node layout of the line-read arrays
-    constexpr int w_a = 4 * n3p + 6 * 7 * n2;         // lifting
+    // FP64 at N = 7: the lifting's in-plane line derivatives on the matrix cores (node_gradient_mfma), with a 4 x 64
+    // scratch per wave after the lifting arrays.  The mixed form keeps them on the VALU: with the scratch (6.6 KB more
+    // LDS) it measured 12 % slower on MI355X.
+    constexpr bool mfma_lift = N == 7 && sizeof(T) == 8;
+    constexpr int w_a0 = 4 * n3p + 6 * 7 * n2;
+    constexpr int w_a = w_a0 + (mfma_lift ? 8 * 4 * n2 : 0);  // lifting
     // mixed split form: the nine metric terms of every node stay in LDS (FP32, halves) in s_geo for the whole
     // kernel, so the volume terms read them there instead of from memory a second time (7.8 % faster at N=7)
     constexpr bool geo_lds = SPLIT && sizeof(T) == 4;
@@ -791,8 +916,15 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
     Stress<T> s;
     {
         T g[12];
-        lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3p, gas, LHm, LHp, g, s_nb,
-                                pgeo ? gm : nullptr);
+        if (mfma_lift) {
+            lift_face_fluxes<N, T, TT>(e, tid, mesh, utr, s_w, s_w + 4 * n3p, gas, s_nb);
+            if (!pgeo) load_node_geo<N, T>(e, tid, mesh, gm);
+            __syncthreads();
+            node_gradient_mfma<T>(tid, M, s_w, s_w + 4 * n3p, s_w + w_a0, LHm, LHp, gm, g);
+        } else {
+            lift_gradient<N, T, TT>(e, tid, mesh, utr, M, s_w, s_w + 4 * n3p, gas, LHm, LHp, g, s_nb,
+                                    pgeo ? gm : nullptr);
+        }
         if (active) stress(pr, g, gas, s);
     }
     __syncthreads();
@@ -812,7 +944,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll
                 for (int v = 0; v < 5; v++) s_f[v * n3p + pt] = f[v];
             }
-            __syncthreads();
+            line_sync<N>(d);
             if (active) {
                 int idx = d == 0 ? i : (d == 1 ? j : k);
                 const auto ln = Lay<N>::line(d, i, j, k);
@@ -824,7 +956,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                     for (int v = 0; v < 5; v++) ut[v] = ut[v] + dh * s_f[v * n3p + o];
                 }
             }
-            __syncthreads();
+            line_sync<N>(d);
         }
     } else {
         // ---- per direction: viscous contravariant flux with the weak-form D_Hat_T contraction
@@ -868,7 +1000,9 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
 #pragma unroll
                 for (int v = 1; v < 5; v++) s_f[(v - 1) * n3p + pt] = f[v];
             }
-            __syncthreads();
+            // s_v (written after the lifting barrier) is read along the lines of d too: the k-lines' barrier below
+            // comes before any read across waves
+            line_sync<N>(d);
             if (active) {
                 int idx = d == 0 ? i : (d == 1 ? j : k);
                 const auto ln = Lay<N>::line(d, i, j, k);
@@ -932,7 +1066,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                 }
                 }
             }
-            __syncthreads();
+            line_sync<N>(d);
         }
 #pragma unroll
         for (int v = 0; v < 5; v++) ut[v] = ut[v] + T(2) * tmp[v];
