@@ -211,12 +211,13 @@ __global__ void __launch_bounds__(256) k_source(long nDOF, int n3, const double*
 
 // per-block sums of the pressure over all nodes (element planes), for the pressure offset of the mixed form; the
 // host adds the block sums in order, so the offset, and with it the mixed-form trajectory, is the same every run
+// per-block sums of the pressure and of the enthalpy H = (E+p)/rho (the mixed form's offsets): out[2b], out[2b+1]
 __global__ void __launch_bounds__(256) k_pressure_sum(long nDOF, int n3, double kappa, const double* __restrict__ U,
                                                       double* __restrict__ out)
 {
-    __shared__ double s[4];
+    __shared__ double s[2][4];
     long x = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double p = 0.0;
+    double p = 0.0, h = 0.0;
     if (x < nDOF) {
         long e = x / n3;
         int node = (int)(x - e * n3);
@@ -224,11 +225,21 @@ __global__ void __launch_bounds__(256) k_pressure_sum(long nDOF, int n3, double 
         double rho = u[0], sRho = 1.0 / rho;
         double m1 = u[n3], m2 = u[2 * n3], m3 = u[3 * n3];
         p = (kappa - 1.0) * (u[4 * n3] - 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) * sRho);
+        h = (u[4 * n3] + p) * sRho;
     }
-    for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
-    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = p;
+    for (int off = 32; off > 0; off >>= 1) {
+        p += __shfl_xor(p, off, 64);
+        h += __shfl_xor(h, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s[0][threadIdx.x >> 6] = p;
+        s[1][threadIdx.x >> 6] = h;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) out[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
+    if (threadIdx.x < 2) {
+        const double* b = s[threadIdx.x];
+        out[2 * blockIdx.x + threadIdx.x] = (b[0] + b[1]) + (b[2] + b[3]);
+    }
 }
 
 // ----------------------------------------------------------------------------------------------------------
@@ -433,6 +444,7 @@ struct galx_ctx {
     Basis basis;
     double lambda = 0.0;
     double pref = 0.0;  // pressure offset of the mixed form (mean initial pressure); 0 in FP64
+    double href = 0.0;  // enthalpy offset of the mixed form's split volume flux (mean initial enthalpy); 0 in FP64
     // device buffers
     double *dU = nullptr, *dK = nullptr, *dUt = nullptr, *dScratch = nullptr;
     double *dGeo = nullptr, *dFgeo = nullptr, *dSgeo = nullptr;
@@ -528,9 +540,12 @@ static Mesh mesh_of(galx_ctx* c)
 {
     return Mesh{c->dGeo, c->dFgeo, c->dConn, c->dInv, c->dSgeo, c->dGeo32, c->dFgeo32, c->dSgeo32, c->dWs};
 }
-static GasD gas_of(galx_ctx* c) { return make_gasd(c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, c->pref); }
+static GasD gas_of(galx_ctx* c)
+{
+    return make_gasd(c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, c->pref, c->href);
+}
 // the adjoint kernels and their FP64 trace records use the pressure itself (no offset)
-static GasD adj_gas_of(galx_ctx* c) { return make_gasd(c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, 0.0); }
+static GasD adj_gas_of(galx_ctx* c) { return make_gasd(c->cfg.kappa, c->cfg.R, c->cfg.mu0, c->lambda, 0.0, 0.0); }
 
 // two pinned host buffers that alternate as staging for host-to-device copies: next() returns a buffer whose
 // previous copy has finished, issued() marks the copies just enqueued from it
@@ -888,20 +903,26 @@ static int after_state(galx_ctx* c)
     HIPCHK(hipMemsetAsync(c->dK, 0, 5L * c->nElems * c->n3 * sizeof(double), c->stream));
     c->cur = 0;
     c->pref = 0.0;
+    c->href = 0.0;
     if (c->cfg.precision == GALX_PREC_MIXED) {
         long nDOF = (long)c->nElems * c->n3;
         const long nb = (nDOF + 255) / 256;
         int rc;
-        if ((rc = dalloc(c, &c->dPsum, nb * sizeof(double)))) return rc;
+        if ((rc = dalloc(c, &c->dPsum, 2 * nb * sizeof(double)))) return rc;
         k_pressure_sum<<<dim3((unsigned)nb), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->cfg.kappa, c->dU, c->dPsum);
         HIPCHK(hipGetLastError());
-        std::vector<double> part(nb);
-        HIPCHK(hipMemcpyAsync(part.data(), c->dPsum, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        std::vector<double> part(2 * nb);
+        HIPCHK(hipMemcpyAsync(part.data(), c->dPsum, 2 * nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
-        double sum = 0.0;
-        for (double v : part) sum += v;
-        c->pref = sum / (double)nDOF;
+        double sp = 0.0, sh = 0.0;
+        for (long b = 0; b < nb; b++) {
+            sp += part[2 * b];
+            sh += part[2 * b + 1];
+        }
+        c->pref = sp / (double)nDOF;
+        c->href = sh / (double)nDOF;
         if (!(c->pref == c->pref)) c->pref = 0.0;
+        if (!(c->href == c->href)) c->href = 0.0;
     }
     {
         TimerScope ts(c, "trace");

@@ -326,6 +326,8 @@ __global__ void __launch_bounds__(Dm<N>::block)
         for (int v = 0; v < 5; v++) {
             T utv = s_ut[v * n3 + node];
             if (SPLIT) utv = utv + T(2) * s_tmp[v * n3 + node];
+            // the enthalpy offset of the split flux (mixed form), as in k_rhs
+            if (SPLIT && sizeof(T) == 4 && v == 4) utv = utv + T(gd.href) * (T(2) * s_tmp[node]);
             T tot = utv + loc[v];
             double utd = double(tot * T(-1)) * sJd;
             if (st.mode == 0) {
