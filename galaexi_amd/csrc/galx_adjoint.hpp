@@ -26,24 +26,24 @@ struct PrimBar {
 
 __device__ __forceinline__ void prim_bar_zero(PrimBar& b) { b.rho = b.u = b.v = b.w = b.p = b.H = b.T = b.E = 0.0; }
 
-// transposed Jacobian of ConsToPrim (eos.f90:212-239) with H = (E+p)/rho and T = p/(rho*R)
-__device__ __forceinline__ void prim_bar_to_cons(const double* c, const PrimBar& b, double kappa, double R,
+// transposed Jacobian of ConsToPrim (eos.f90:212-239) with H = (E+p)/rho and T = p/(rho*R); rR = 1/R
+__device__ __forceinline__ void prim_bar_to_cons(const double* c, const PrimBar& b, double kappa, double rR,
                                                  double* Ubar)
 {
-    double rho = c[0], sRho = 1.0 / rho;
+    double rho = c[0], sRho = rcp64(rho);
     double u = c[1] * sRho, v = c[2] * sRho, w = c[3] * sRho;
     double q2 = 0.5 * (u * u + v * v + w * w);
     double km1 = kappa - 1.0;
     double p = km1 * (c[4] - rho * q2);
-    double H = (c[4] + p) * sRho, T = p * sRho / R;
+    double H = (c[4] + p) * sRho, T = p * sRho * rR;
     double dp_drho = km1 * q2;
     Ubar[0] = b.rho - (b.u * u + b.v * v + b.w * w) * sRho + b.p * dp_drho + b.H * (dp_drho - H) * sRho +
-              b.T * (dp_drho * sRho / R - T * sRho);
-    double s = -km1 * (b.p + b.H * sRho + b.T * sRho / R);
+              b.T * (dp_drho * sRho * rR - T * sRho);
+    double s = -km1 * (b.p + b.H * sRho + b.T * sRho * rR);
     Ubar[1] = b.u * sRho + s * u;
     Ubar[2] = b.v * sRho + s * v;
     Ubar[3] = b.w * sRho + s * w;
-    Ubar[4] = b.E + b.p * km1 + b.H * kappa * sRho + b.T * km1 * sRho / R;
+    Ubar[4] = b.E + b.p * km1 + b.H * kappa * sRho + b.T * km1 * sRho * rR;
 }
 
 // Adjoint of the viscous flux -m.(tau, tau.u + lambda grad T) (EvalNormalDiffFlux3D flux.f90:779-847 for m = n,
@@ -122,15 +122,16 @@ __device__ __forceinline__ void lf_pi_adjoint(const double* UL, const double* UR
                                               double kappa, int side, PrimBar& pb)
 {
     const double* Us[2] = {UL, UR};
-    double rho[2], vn[2], p[2], H[2], c[2];
+    double rho[2], vn[2], p[2], H[2], c[2], srs[2];
     double us[3];  // velocity of the differentiated side
     double usum[3] = {0, 0, 0};
 #pragma unroll
     for (int s = 0; s < 2; s++) {
         const double* c5 = Us[s];
-        double sr = 1.0 / c5[0];
+        double sr = rcp64(c5[0]);
         double u = c5[1] * sr, v = c5[2] * sr, w = c5[3] * sr;
         rho[s] = c5[0];
+        srs[s] = sr;
         p[s] = (kappa - 1.0) * (c5[4] - 0.5 * (c5[1] * u + c5[2] * v + c5[3] * w));
         H[s] = (c5[4] + p[s]) * sr;
         c[s] = sqrt(kappa * p[s] * sr);
@@ -168,8 +169,8 @@ __device__ __forceinline__ void lf_pi_adjoint(const double* UL, const double* UR
     const double wv = max_weight(fabs(vn[side]), fabs(vn[1 - side]));
     const double wc = max_weight(c[side], c[1 - side]);
     bvn += blmax * wv * sign0(vn[side]);
-    bp += blmax * wc * kappa / (2.0 * c[side] * rho[side]);
-    brho += -blmax * wc * c[side] / (2.0 * rho[side]);
+    bp += blmax * wc * kappa * (0.5 * rcp64(c[side])) * srs[side];
+    brho += -blmax * wc * c[side] * (0.5 * srs[side]);
     pb.rho += brho;
     pb.u += bu[0] + bvn * nv[0];
     pb.v += bu[1] + bvn * nv[1];
@@ -239,7 +240,7 @@ __device__ __forceinline__ void lift_face_node_ubar_at(int tid, const double* c,
     pb.v = qb[1];
     pb.w = qb[2];
     pb.T = qb[3];
-    prim_bar_to_cons(c, pb, gd.kappa, gd.R, Ub);
+    prim_bar_to_cons(c, pb, gd.kappa, gd.rR, Ub);
 }
 
 template <int N>
@@ -514,7 +515,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
             Un[1] = tr[n2];
             Un[2] = tr[2 * n2];
             Un[3] = tr[3 * n2];
-            Un[4] = tr[4 * n2] / (gd.kappa - 1.0) + 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]) / Un[0];
+            Un[4] = tr[4 * n2] * gd.rkm1 + 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]) * rcp64(Un[0]);
         }
         const double* fg = mesh.fgeo + side * 10 * n2 + pq;
         double nv[3] = {fg[0], fg[n2], fg[2 * n2]}, se = fg[9 * n2];
@@ -540,7 +541,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
         fp.v += uvw[1];
         fp.w += uvw[2];
         double Ub[5];
-        prim_bar_to_cons(Uo, fp, gd.kappa, gd.R, Ub);
+        prim_bar_to_cons(Uo, fp, gd.kappa, gd.rR, Ub);
         double* o = s_r + (ls - 1) * 12 * n2 + ab;
 #pragma unroll
         for (int v = 0; v < 5; v++) o[v * n2] = Ub[v];
@@ -628,7 +629,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
 #pragma unroll
     for (int v = 0; v < 5; v++) cU[v] = u[v * n3];
     double Ubv[5];
-    prim_bar_to_cons(cU, pb, gd.kappa, gd.R, Ubv);
+    prim_bar_to_cons(cU, pb, gd.kappa, gd.rR, Ubv);
     double* ub = Ubar + (long)e * 5 * n3 + tid;
     double* lp = lam + (long)e * 5 * n3 + tid;
 #pragma unroll
