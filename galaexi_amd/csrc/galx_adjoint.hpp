@@ -413,7 +413,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
                 s_m[n3 + tid] = mme[1];
                 s_m[2 * n3 + tid] = mme[2];
             }
-            __syncthreads();
+            line_sync<N>(d);  // s_v, s_mu and s_m are read along the lines of d
             if (active) {
                 int idx = d == 0 ? i : (d == 1 ? j : k);
                 int stride = d == 0 ? 1 : (d == 1 ? n : n2);
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
                     pb.H += q * wv[4];
                 }
             }
-            __syncthreads();
+            line_sync<N>(d);
         }
     }
     // ---- forward lifting, then the adjoint of the weak viscous volume term (VolInt_weakForm_Visc,
@@ -606,7 +606,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
 #pragma unroll
             for (int q = 0; q < 4; q++) s_a[q * n3 + tid] = m0 * Gb[q] + m1 * Gb[4 + q] + m2 * Gb[8 + q];
         }
-        __syncthreads();
+        line_sync<N>(d);
         if (active) {
             int idx = d == 0 ? i : (d == 1 ? j : k);
             int stride = d == 0 ? 1 : (d == 1 ? n : n2);
@@ -617,7 +617,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
                 for (int q = 0; q < 4; q++) qb[q] += dd * s_a[q * n3 + base + m * stride];
             }
         }
-        __syncthreads();
+        line_sync<N>(d);
     }
     if (!active) return;
     pb.u += qb[0];
