@@ -371,6 +371,8 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
     __shared__ double s_mu[5 * n3];
     __shared__ double s_tau[6 * 6 * n2];  // own stress (txx,tyy,tzz,txy,txz,tyz) at the face nodes
     __shared__ Mats<N, double, false> M;  // column-major: the adjoint contractions read (idx, m)
+    // the i- and j-lines inside one 64-lane wave (N = 1, 3, 7): shared pairs in the split adjoint, as in k_rhs
+    constexpr bool pair_lines = 64 % n == 0 && 64 % n2 == 0;
     const int e = blockIdx.x, tid = threadIdx.x;
     const bool active = tid < n3;
     const int i = tid % n, j = (tid / n) % n, k = tid / n2;
@@ -418,11 +420,12 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
                 int idx = d == 0 ? i : (d == 1 ? j : k);
                 int stride = d == 0 ? 1 : (d == 1 ? n : n2);
                 int base = tid - idx * stride;
-                // l = idx included like the forward: F#(U,U) has equal partial derivatives in its two
-                // arguments, so the same formula gives 2 DVS(i,i) dF#/dU_i^T mu_i
-                for (int l = 0; l <= N; l++) {
-                    int o = base + l * stride;
-                    double dli = M.dvs(l, idx), dil = M.dvs(idx, l);
+                // the increment of the primitive adjoint (rho,u,v,w,p,H) of pair (idx, l).  F# depends on the
+                // two states only through their sums and the metric sums, so both nodes of the pair receive the
+                // same increment (wv, and with it everything below, is symmetric in the pair)
+                auto pair_bar = [&](int l, double* P) {
+                    const int o = base + l * stride;
+                    const double dli = M.dvs(l, idx), dil = M.dvs(idx, l);
                     double wv[5];
 #pragma unroll
                     for (int v = 0; v < 5; v++) wv[v] = dli * s_mu[v * n3 + tid] + dil * s_mu[v * n3 + o];
@@ -433,12 +436,49 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
                     double q = 0.125 * sr * ms;
                     double A = 2.0 * wv[0] + su * wv[1] + sv * wv[2] + sw * wv[3] + sH * wv[4];
                     double c8 = 0.125 * sr * A;
-                    pb.rho += 0.125 * ms * A;
-                    pb.u += c8 * m0 + q * wv[1];
-                    pb.v += c8 * m1 + q * wv[2];
-                    pb.w += c8 * m2 + q * wv[3];
-                    pb.p += 0.5 * (m0 * wv[1] + m1 * wv[2] + m2 * wv[3]);
-                    pb.H += q * wv[4];
+                    P[0] = 0.125 * ms * A;
+                    P[1] = c8 * m0 + q * wv[1];
+                    P[2] = c8 * m1 + q * wv[2];
+                    P[3] = c8 * m2 + q * wv[3];
+                    P[4] = 0.5 * (m0 * wv[1] + m1 * wv[2] + m2 * wv[3]);
+                    P[5] = q * wv[4];
+                };
+                auto add_bar = [&](const double* P) {
+                    pb.rho += P[0];
+                    pb.u += P[1];
+                    pb.v += P[2];
+                    pb.w += P[3];
+                    pb.p += P[4];
+                    pb.H += P[5];
+                };
+                if (pair_lines && d < 2) {
+                    // lines inside one wave, as the forward's shared pairs: the lane at line position idx forms
+                    // the pairs (idx, idx+s) for s = 1..(n-1)/2 (mod n) and takes (idx-s, idx) from the lane s
+                    // positions back; for even n both ends form (idx, idx+n/2).  The diagonal is left out, as in
+                    // the forward's shared pairs.
+                    constexpr int msk = n - 1;
+                    const int lane0 = (tid & 63) - idx * stride;  // node = thread: lane distance = node stride
+#pragma unroll
+                    for (int s = 1; s <= n / 2; s++) {
+                        double P[6];
+                        pair_bar((idx + s) & msk, P);
+                        add_bar(P);
+                        if (2 * s < n) {
+                            const int src = lane0 + ((idx - s) & msk) * stride;
+                            double R[6];
+#pragma unroll
+                            for (int c = 0; c < 6; c++) R[c] = __shfl(P[c], src);
+                            add_bar(R);
+                        }
+                    }
+                } else {
+                    // l = idx included like the forward: F#(U,U) has equal partial derivatives in its two
+                    // arguments, so the same formula gives 2 DVS(i,i) dF#/dU_i^T mu_i
+                    for (int l = 0; l <= N; l++) {
+                        double P[6];
+                        pair_bar(l, P);
+                        add_bar(P);
+                    }
                 }
             }
             line_sync<N>(d);
