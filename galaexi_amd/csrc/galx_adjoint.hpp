@@ -3,14 +3,16 @@
 // For Ut = R(U) (split Pirozzoli volume flux, Lax-Friedrichs surface flux with the PI central part, BR1 lifting,
 // constant viscosity, FP64) and a cotangent lam = dJ/dUt, the kernels form Ubar = (dR/dU)^T lam exactly (to
 // rounding), element per workgroup and without atomics: every element gathers what its own nodes receive.
-//   k_adj_pre  : lam = kbar + bdt*Ubar (LSERK stage adjoint, in place), mu = -sJ*lam on the faces scaled by the
+//   k_adj_pre  : lam = akbar*kbar + bdt*Ubar (LSERK stage adjoint, in place; akbar = the previous stage's register
+//                coefficient, kbar = a*lam of the stage after, applied here instead of by a pass of its own), mu =
+//                -sJ*lam on the faces scaled by the
 //                surface lifting weight L_Hat of the writer's local side -> mu traces; in a sweep it first adds
 //                the pending neighbour lifting term of the previous VJP (so k_adj_face runs once per sweep)
 //   k_adj_main : forward recompute (primitives, lifted gradients), adjoint of the weak viscous volume term
 //                (D_Hat^T transposed), of the split volume fluxes (two-point flux Jacobians transposed and
 //                gathered along the lines), of the face fluxes (transposed LF/PI and viscous Jacobians), of the
 //                BR1 lifting (transposed strong-form derivative, own surface term), chained through ConsToPrim;
-//                Ubar (+)= result, kbar = a*lam, and the traces L_Hat * n.ghat (ghat = sJ*Gbar) for the
+//                Ubar (+)= result and the traces L_Hat * n.ghat (ghat = sJ*Gbar) for the
 //                neighbours' lifting adjoint
 //   k_adj_face : the neighbour's lifting surface term on the face nodes
 // The forward operator is the one of k_lift/k_rhs (src/dg/dg.f90:255-425), whose pieces are cited there.
@@ -258,7 +260,7 @@ __device__ __forceinline__ bool lift_face_node_ubar(int e, int tid, const double
 }
 
 // ---------------------------------------------------------------------------------------------------------
-// k_adj_pre: lam = kbar + bdt*Ubar (kbar updated in place when Ubar is given), mu traces
+// k_adj_pre: lam = akbar*kbar + bdt*Ubar (kbar updated in place when Ubar is given), mu traces
 //   mutr [side][2 slots][5][n2] = L_Hat(locSide) * (-sJ * lam) at the face nodes of the writer (the same
 //   product, in the same order, as k_adj_main forms for its own side).
 //   Uprev (optional, with Ubar): first completes the VJP of the stage before (in reverse order), evaluated at
@@ -266,8 +268,9 @@ __device__ __forceinline__ bool lift_face_node_ubar(int e, int tid, const double
 // ---------------------------------------------------------------------------------------------------------
 template <int N>
 __global__ void __launch_bounds__(Dm<N>::block)
-    k_adj_pre(double* __restrict__ kbar, double* __restrict__ Ubar, double bdt, Mesh mesh, double LHm, double LHp,
-              double* __restrict__ mutr, const double* __restrict__ Uprev, const double* __restrict__ nghtr, GasD gd)
+    k_adj_pre(double* __restrict__ kbar, double* __restrict__ Ubar, double akbar, double bdt, Mesh mesh, double LHm,
+              double LHp, double* __restrict__ mutr, const double* __restrict__ Uprev, const double* __restrict__ nghtr,
+              GasD gd)
 {
     using D = Dm<N>;
     static_assert(D::node_per_thread, "one volume node per thread: N <= 9");
@@ -309,7 +312,7 @@ __global__ void __launch_bounds__(Dm<N>::block)
         }
 #pragma unroll
         for (int v = 0; v < 5; v++) {
-            double l = Ubar ? kb5[v] + bdt * u5[v] : kb5[v];
+            double l = Ubar ? akbar * kb5[v] + bdt * u5[v] : kb5[v];
             if (Ubar) kbar[off + v * n3] = l;
             s_l[v * n3 + tid] = msJ * l;
         }
@@ -331,9 +334,8 @@ __global__ void __launch_bounds__(Dm<N>::block)
 
 // ---------------------------------------------------------------------------------------------------------
 // k_adj_main (split PI volume flux, LF surface flux, FP64)
-//   in : U, U trace records utr (of this U), lam, mu traces; accumulate: Ubar += result, else Ubar = result;
-//        kscale: if set, lam = a*lam at the end (the LSERK register adjoint, in place)
-//   out: Ubar, lam, nghtr [side][2][4][n2] = L_Hat(locSide) * (n_master . sJ*Gbar) of the writer
+//   in : U, U trace records utr (of this U), lam, mu traces; accumulate: Ubar += result, else Ubar = result
+//   out: Ubar, nghtr [side][2][4][n2] = L_Hat(locSide) * (n_master . sJ*Gbar) of the writer
 // Phases: split volume adjoint, forward lifting with the viscous volume adjoint, face adjoint, lifting adjoint.
 // LDS: one work array reused by the phases, mu = -sJ*lam, the own stress at the face nodes and the matrices
 // (76.5 KB at N=7: two workgroups per CU, with AdjWaves capping the registers to match).
@@ -359,9 +361,9 @@ struct AdjWaves {
 
 template <int N>
 __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
-    k_adj_main(const double* __restrict__ U, Mesh mesh, const double* __restrict__ utr, double* __restrict__ lam,
+    k_adj_main(const double* __restrict__ U, Mesh mesh, const double* __restrict__ utr, const double* __restrict__ lam,
                const double* __restrict__ mutr, const double* __restrict__ mats, GasD gd, double LHm, double LHp,
-               double* __restrict__ Ubar, int accumulate, int kscale, double a, double* __restrict__ nghtr)
+               double* __restrict__ Ubar, int accumulate, double* __restrict__ nghtr)
 {
     using D = Dm<N>;
     static_assert(D::node_per_thread, "one volume node per thread: N <= 9");
@@ -664,6 +666,8 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
     pb.v += qb[1];
     pb.w += qb[2];
     pb.T += qb[3];
+    // U read again: kept in registers since the load (the primitives), it spilled 32 more bytes per lane and
+    // measured 3 % slower on MI355X
     double cU[5];
     const double* u = U + (long)e * 5 * n3 + tid;
 #pragma unroll
@@ -671,12 +675,10 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
     double Ubv[5];
     prim_bar_to_cons(cU, pb, gd.kappa, gd.rR, Ubv);
     double* ub = Ubar + (long)e * 5 * n3 + tid;
-    double* lp = lam + (long)e * 5 * n3 + tid;
 #pragma unroll
     for (int v = 0; v < 5; v++) {
         double r = Ubv[v] + Ubf[v];
         ub[v * n3] = accumulate ? ub[v * n3] + r : r;
-        if (kscale) lp[v * n3] = a * lp[v * n3];
     }
 }
 

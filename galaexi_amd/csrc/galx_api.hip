@@ -344,10 +344,10 @@ struct Launch {
 // discrete adjoint (split PI + LF, FP64)
 struct AdjSet {
     bool ok = false;
-    void (*pre)(hipStream_t, int, double*, double*, double, Mesh, double, double, double*, const double*,
+    void (*pre)(hipStream_t, int, double*, double*, double, double, Mesh, double, double, double*, const double*,
                 const double*, GasD) = nullptr;
-    void (*main)(hipStream_t, int, const double*, Mesh, const double*, double*, const double*, const double*, GasD,
-                 double, double, double*, int, int, double, double*) = nullptr;
+    void (*main)(hipStream_t, int, const double*, Mesh, const double*, const double*, const double*, const double*,
+                 GasD, double, double, double*, int, double*) = nullptr;
     void (*face)(hipStream_t, int, const double*, Mesh, const double*, GasD, double*) = nullptr;
     void (*trace64)(hipStream_t, int, const double*, Mesh, double*, GasD) = nullptr;  // FP64 U trace records
 };
@@ -355,18 +355,18 @@ struct AdjSet {
 template <int N>
 struct AdjLaunch {
     using D = Dm<N>;
-    static void pre(hipStream_t s, int nE, double* kbar, double* Ubar, double bdt, Mesh m, double LHm, double LHp,
-                    double* mutr, const double* Uprev, const double* nghtr, GasD g)
+    static void pre(hipStream_t s, int nE, double* kbar, double* Ubar, double akbar, double bdt, Mesh m, double LHm,
+                    double LHp, double* mutr, const double* Uprev, const double* nghtr, GasD g)
     {
-        hipLaunchKernelGGL((k_adj_pre<N>), dim3(nE), dim3(D::block), 0, s, kbar, Ubar, bdt, m, LHm, LHp, mutr, Uprev,
-                           nghtr, g);
+        hipLaunchKernelGGL((k_adj_pre<N>), dim3(nE), dim3(D::block), 0, s, kbar, Ubar, akbar, bdt, m, LHm, LHp, mutr,
+                           Uprev, nghtr, g);
     }
-    static void main(hipStream_t s, int nE, const double* U, Mesh m, const double* utr, double* lam,
+    static void main(hipStream_t s, int nE, const double* U, Mesh m, const double* utr, const double* lam,
                      const double* mutr, const double* mats, GasD g, double LHm, double LHp, double* Ubar, int acc,
-                     int kscale, double a, double* nghtr)
+                     double* nghtr)
     {
         hipLaunchKernelGGL((k_adj_main<N>), dim3(nE), dim3(D::block), 0, s, U, m, utr, lam, mutr, mats, g, LHm, LHp,
-                           Ubar, acc, kscale, a, nghtr);
+                           Ubar, acc, nghtr);
     }
     static void face(hipStream_t s, int nE, const double* U, Mesh m, const double* nghtr, GasD g, double* Ubar)
     {
@@ -1356,20 +1356,21 @@ static const double* adj_traces(galx_ctx* c, const double* U, const double* fp64
 static const double* own_fp64_traces(galx_ctx* c) { return c->tsz == 8 ? (const double*)c->dUtr[c->cur] : nullptr; }
 
 // Ubar (+)= (dR/dU(U))^T lam for U in element planes with its FP64 trace records utr; lam is consumed:
-// lam <- lam + bdt*Ubar first when Ubar accumulates (LSERK stage), and lam <- a*lam at the end when kscale.
+// lam <- akbar*lam + bdt*Ubar first when Ubar accumulates (LSERK stage: akbar is the register coefficient a of the
+// stage reversed before, whose kbar = a*lam_tot this applies).
 // Uprev: state of the previous VJP whose neighbour lifting term is still pending (completed first, fused);
 // finish: apply this VJP's own neighbour lifting term now (else it stays pending for the next call).
 static void adj_apply(galx_ctx* c, const double* U, const double* utr, double* lam, double* Ubar, int accumulate,
-                      double bdt, int kscale, double a, const double* Uprev, bool finish)
+                      double bdt, double akbar, const double* Uprev, bool finish)
 {
     TimerScope ts(c, "adjoint");
     const Basis& B = c->basis;
     double LHm = 1.0 / B.w[0], LHp = 1.0 / B.w[c->N];
     Mesh m = mesh_of(c);
     const GasD g = adj_gas_of(c);
-    c->as.pre(c->stream, c->nElems, lam, accumulate ? Ubar : nullptr, bdt, m, LHm, LHp, c->dMutr, Uprev, c->dNghtr,
-              g);
-    c->as.main(c->stream, c->nElems, U, m, utr, lam, c->dMutr, c->dMats64, g, LHm, LHp, Ubar, accumulate, kscale, a,
+    c->as.pre(c->stream, c->nElems, lam, accumulate ? Ubar : nullptr, akbar, bdt, m, LHm, LHp, c->dMutr, Uprev,
+              c->dNghtr, g);
+    c->as.main(c->stream, c->nElems, U, m, utr, lam, c->dMutr, c->dMats64, g, LHm, LHp, Ubar, accumulate,
                c->dNghtr);
     if (finish) c->as.face(c->stream, c->nElems, U, m, c->dNghtr, g, Ubar);
 }
@@ -1383,7 +1384,7 @@ int galx_rhs_vjp_device(galx_ctx* c, const double* dLam, double* dUbarOut)
     unsigned g = (unsigned)((nDOF + 255) / 256);
     k_ref_to_planes<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, dLam, c->dKbar);
     HIPCHK(hipGetLastError());
-    adj_apply(c, c->dU, adj_traces(c, c->dU, own_fp64_traces(c)), c->dKbar, c->dUbar, 0, 0.0, 0, 0.0, nullptr, true);
+    adj_apply(c, c->dU, adj_traces(c, c->dU, own_fp64_traces(c)), c->dKbar, c->dUbar, 0, 0.0, 1.0, nullptr, true);
     HIPCHK(hipGetLastError());
     k_planes_to_ref<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUbar, dUbarOut);
     HIPCHK(hipGetLastError());
@@ -1402,7 +1403,7 @@ int galx_rhs_vjp(galx_ctx* c, const double* lam, double* UbarOut)
     HIPCHK(hipMemcpyAsync(c->dScratch, lam, 5 * nDOF * sizeof(double), hipMemcpyHostToDevice, c->stream));
     k_ref_to_planes<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dScratch, c->dKbar);
     HIPCHK(hipGetLastError());
-    adj_apply(c, c->dU, adj_traces(c, c->dU, own_fp64_traces(c)), c->dKbar, c->dUbar, 0, 0.0, 0, 0.0, nullptr, true);
+    adj_apply(c, c->dU, adj_traces(c, c->dU, own_fp64_traces(c)), c->dKbar, c->dUbar, 0, 0.0, 1.0, nullptr, true);
     HIPCHK(hipGetLastError());
     k_planes_to_ref<<<dim3(g), dim3(256), 0, c->stream>>>(nDOF, c->n3, c->dUbar, c->dScratch);
     HIPCHK(hipGetLastError());
@@ -1494,6 +1495,7 @@ static int adj_reverse_steps(galx_ctx* c)
     const int C = c->trajEvery, nSteps = c->trajSteps;
     const int nSeg = (nSteps + C - 1) / C;
     const double* pending = nullptr;
+    double akbar = 1.0;  // the register coefficient a still to be applied to kbar (kbar is zero at the start)
     for (int seg = nSeg - 1; seg >= 0; seg--) {
         const int s0 = seg * C, n = std::min(C, nSteps - s0);
         auto step_in = [&](int s) -> double* { return s == 0 ? c->ckpt[seg] : c->steps[s - 1]; };
@@ -1511,8 +1513,9 @@ static int adj_reverse_steps(galx_ctx* c)
             for (int stage = 4; stage >= 0; stage--) {
                 const double* Us = stage == 0 ? step_in(s) : c->stages[stage - 1];
                 const double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
-                adj_apply(c, Us, adj_traces(c, Us, nullptr), c->dKbar, c->dUbar, 1, RKB[stage] * c->trajDt, 1, mRKA,
+                adj_apply(c, Us, adj_traces(c, Us, nullptr), c->dKbar, c->dUbar, 1, RKB[stage] * c->trajDt, akbar,
                           pending, false);
+                akbar = mRKA;
                 // the pending state is read by the next VJP: a stage buffer of this step or a step input, neither
                 // overwritten before that
                 pending = Us;
@@ -1535,6 +1538,7 @@ static int adj_reverse(galx_ctx* c)
     Mesh m = mesh_of(c);
     GasD gd = gas_of(c);
     const double* pending = nullptr;  // state of the last VJP whose neighbour lifting term is still to be added
+    double akbar = 1.0;               // the register coefficient a still to be applied to kbar (zero at the start)
     HIPCHK(hipMemsetAsync(c->dKbar, 0, ub, c->stream));
     if (c->adjStorage == GALX_ADJ_STEPS) return adj_reverse_steps(c);
     for (int seg = nSeg - 1; seg >= 0; seg--) {
@@ -1566,7 +1570,8 @@ static int adj_reverse(galx_ctx* c)
                 const double* Us = idx == 0 ? c->ckpt[seg] : c->stages[idx];
                 double mRKA = stage == 0 ? 0.0 : -1.0 * RKA[stage];
                 const double* tr = adj_traces(c, Us, keep ? c->stageTr[idx] : nullptr);
-                adj_apply(c, Us, tr, c->dKbar, c->dUbar, 1, RKB[stage] * c->trajDt, 1, mRKA, pending, false);
+                adj_apply(c, Us, tr, c->dKbar, c->dUbar, 1, RKB[stage] * c->trajDt, akbar, pending, false);
+                akbar = mRKA;
                 pending = Us;  // a checkpoint or a stage buffer of this segment, read before it is overwritten
             }
         HIPCHK(hipGetLastError());
