@@ -270,13 +270,13 @@ __device__ __forceinline__ FaceTask face_task(int e, int t, const Mesh& mesh)
 
 // the memory operands of a lifting face task: the neighbour's trace record, the normal and SurfElem
 template <typename T>
-struct LiftPre {
+struct LiftOperands {
     T tr[5], nv[3], se;
 };
 
 template <int N, typename T, typename TT>
-__device__ __forceinline__ void lift_prefetch(const FaceTask& f, const Mesh& mesh, const TT* __restrict__ utr,
-                                              LiftPre<T>& p)
+__device__ __forceinline__ void load_lift_operands(const FaceTask& f, const Mesh& mesh, const TT* __restrict__ utr,
+                                                   LiftOperands<T>& p)
 {
     constexpr int n2 = (N + 1) * (N + 1);
     const TT* nb = utr + ((f.side * 2 + (f.flip ? 0 : 1)) * 5) * n2 + f.pq;
@@ -292,8 +292,8 @@ __device__ __forceinline__ void lift_prefetch(const FaceTask& f, const Mesh& mes
 // BR1 lifting flux 0.5*SurfElem*(-L_master + L_slave) and the normal of one face node (s_lflux [6][7][n2]); the
 // neighbour's trace record also to s_nb [6][5][n2] when given
 template <int N, typename T>
-__device__ __forceinline__ void lift_face_task(const FaceTask& f, const LiftPre<T>& p, const T* s_lift, T* s_lflux,
-                                               const Gas<T>& gas, T* s_nb)
+__device__ __forceinline__ void lift_face_task(const FaceTask& f, const LiftOperands<T>& p, const T* s_lift,
+                                               T* s_lflux, const Gas<T>& gas, T* s_nb)
 {
     constexpr int n = N + 1, n2 = n * n;
     const int node = face_vol_node<N>(f.ls, f.ab % n, f.ab / n);
@@ -323,8 +323,8 @@ __device__ __forceinline__ void lift_face_fluxes(int e, int tid, const Mesh& mes
 {
     for (int t = tid; t < Dm<N>::nf; t += Dm<N>::block) {
         const FaceTask f = face_task<N>(e, t, mesh);
-        LiftPre<T> p;
-        lift_prefetch<N, T, TT>(f, mesh, utr, p);
+        LiftOperands<T> p;
+        load_lift_operands<N, T, TT>(f, mesh, utr, p);
         lift_face_task<N, T>(f, p, s_lift, s_lflux, gas, s_nb);
     }
 }
