@@ -1,4 +1,4 @@
-"""Two benchmark configurations on the GPU at their own sizes.
+"""Three benchmark configurations on the GPU at their own sizes.
 
 C1: isentropic vortex (ExactFunc CASE(7), exactfunc.f90:482-503) with seeded centre in [-1,1]^2 and amplitude in
 [0.5,1], base flow rho=1, p=1/kappa, u=(0.5,0,0) (Ma 0.5), on the periodic box [-5,5]^3 with 8^3 elements, N=3
@@ -6,7 +6,7 @@ Gauss-Lobatto, weak form, Roe, BR1, mu0=1e-4, R=1, Pr=0.72, CFLScale=DFLScale=0.
 held fixed, 100 LSERK steps.  The whole run against the FP64 oracle's run: relative L2 <= 1e-12 after 500 operator
 evaluations (measured on MI355X: 1.0e-15).
 
-C2: TGV Re1600 Ma0.1 on [0,2pi]^3 with 32^3 elements, N=7, split PI + LF, BR1, seeded 1e-3 velocity noise.  Too
+C2 (and C3, the headline mesh, below): TGV Re1600 Ma0.1 on [0,2pi]^3 with 32^3 elements, N=7, split PI + LF, BR1, seeded 1e-3 velocity noise.  Too
 large for the oracle at test speed; the size-independent property is discrete conservation.  On a periodic mesh
 sum_i w_i J_i Ut_i = 0 for every conservative variable, so after 20 steps the integrals of rho, rho*u, rho*v, rho*w
 and E stay constant to rounding.  Bar against each one's natural scale (int rho, int rho|u|, int E): 1e-13
@@ -89,3 +89,50 @@ def test_c2_tgv_full_size_conservation():
     say(f"  drift of the integrals over 20 steps (relative to their scales): {['%.1e' % d for d in drift]}")
     assert np.all(drift <= 1e-13)
     assert np.abs(U1 - U0).max() > 1e-6  # the state did evolve
+
+
+def _integrals(m, U, w):
+    W = w[:, None, None, None] * w[None, :, None, None] * w[None, None, :, None] / m.sJ
+    I = np.array([np.sum(W * U[v]) for v in range(5)])
+    speed = np.sqrt(U[1] ** 2 + U[2] ** 2 + U[3] ** 2)
+    scale = np.array([np.sum(W * U[0]), *(3 * [np.sum(W * speed)]), np.sum(W * U[4])])
+    return I, scale
+
+
+@pytest.fixture(scope="module")
+def c3_case():
+    """the C3 mesh and initial state, built once for both forms (about 45 s of host work)"""
+    m = M.box_mesh((48, 48, 48), 7, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3)
+    return m, T.taylor_green(m.Elem_xGP, Ma=0.1, noise=1e-3, seed=1600)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_c3_tgv_headline_size_conservation(precision, c3_case):
+    """C3, the headline mesh: TGV Re1600 Ma0.1, 48^3 elements, N=7 (56.6M DOF), split PI + LF, BR1, in both forms,
+    10 LSERK steps at the CALCTIMESTEP dt.  Discrete conservation of the five integrals, as for C2.  FP64: 1e-12 of
+    each integral's scale (measured on MI355X: 1.6e-13 for rho, 3.1e-15 for the momenta, 1.8e-15 for E; rounding of
+    56.6M nodal residuals and their weights).  Mixed: 1e-11 (measured: 2.4e-13 for rho, 8.6e-14..2.3e-13 for the
+    momenta, 1.1e-13 for E).  Its face fluxes are the same FP32 values on both sides and the split pair fluxes are
+    symmetric bit for bit; the FP32 rounding of the nodal residuals is random in sign and averages out of the
+    integrals."""
+    from galaexi_amd import dg
+
+    say(f"case: C3 TGV 48^3 N=7 split LF {precision}, 10 steps, conservation of the five integrals")
+    m, U0 = c3_case
+    op = dg.DGOperator(7, form="split", riemann="lf", precision=precision, kappa=1.4, R=71.42857, Pr=0.72,
+                       mu0=1.0 / 1600.0, CFLScale=0.5, DFLScale=0.5)
+    op.set_mesh(m)
+    op.set_state(U0)
+    dt, err = op.CalcTimeStep()
+    assert err == 0
+    op.TimeStepByLSERKW2(0.0, dt, nSteps=10)
+    U1 = op.get_state()
+    op.close()
+    w = O.dg_basis(7)["wGP"]
+    I0, scale = _integrals(m, U0, w)
+    I1, _ = _integrals(m, U1, w)
+    drift = np.abs(I1 - I0) / scale
+    say(f"  drift of the integrals over 10 steps (relative to their scales): {['%.1e' % d for d in drift]}")
+    assert np.all(np.isfinite(U1))
+    assert np.abs(U1 - U0).max() > 1e-6  # the state did evolve
+    assert np.all(drift <= (1e-12 if precision == "fp64" else 1e-11))
