@@ -391,9 +391,46 @@ __device__ __forceinline__ void gradient_combine(int node, const T* s_lflux, T L
     }
 }
 
+// the same with fewer live registers (k_lift): each direction's derivatives become the Cartesian sums as they are
+// formed, and the surface terms are added to those sums face by face, with no separate surface accumulator.  The
+// sums round in another order than gradient_combine's, within the FP64 parity bar.
+template <int N, typename T>
+__device__ __forceinline__ void gradient_combine_lean(int node, const T* s_lflux, T LHm, T LHp, const T* gm, T* gxi,
+                                                      T* get, T* gze, T* g)
+{
+    constexpr int n = N + 1, n2 = n * n;
+    const int i = node % n, j = (node / n) % n, k = node / n2;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const T a = gxi[q], b = get[q], c = gze[q];
+        g[q] = gm[0] * a + gm[3] * b + gm[6] * c;
+        g[4 + q] = gm[1] * a + gm[4] * b + gm[7] * c;
+        g[8 + q] = gm[2] * a + gm[5] * b + gm[8] * c;
+    }
+#pragma unroll
+    for (int ls = 1; ls <= 6; ls++) {
+        int ab;
+        if (!on_face<N>(ls, i, j, k, ab)) continue;
+        const T* o = s_lflux + (ls - 1) * 7 * n2 + ab;
+        T LH = (ls == 1 || ls == 2 || ls == 5) ? LHm : LHp;
+        T n0 = o[4 * n2], n1 = o[5 * n2], n2v = o[6 * n2];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            T lf = o[q * n2];
+            g[q] = g[q] + (lf * n0) * LH;
+            g[4 + q] = g[4 + q] + (lf * n1) * LH;
+            g[8 + q] = g[8 + q] + (lf * n2v) * LH;
+        }
+    }
+    const T sJ = gm[9];
+#pragma unroll
+    for (int q = 0; q < 12; q++) g[q] = sJ * g[q];
+}
+
 // gm: the node's Ja^1(3), Ja^2(3), Ja^3(3), sJ (load_node_geo), loaded by the caller ahead of the barrier that
-// precedes this call so that their latency overlaps the face tasks
-template <int N, typename T, class MT>
+// precedes this call so that their latency overlaps the face tasks.  LEAN: the line loop one step at a time and
+// gradient_combine_lean, so that k_lift fits 80 registers without spilling (three FP64 workgroups per CU)
+template <int N, typename T, class MT, bool LEAN = false>
 __device__ __forceinline__ void node_gradient(int node, const MT& M, const T* s_lift, const T* s_lflux,
                                               T LHm, T LHp, const T* gm, T* g)
 {
@@ -401,8 +438,7 @@ __device__ __forceinline__ void node_gradient(int node, const MT& M, const T* s_
     constexpr int n3p = Lay<N>::n3;
     const int i = node % n, j = (node / n) % n, k = node / n2;
     T gxi[4] = {0, 0, 0, 0}, get[4] = {0, 0, 0, 0}, gze[4] = {0, 0, 0, 0};
-#pragma unroll GALX_UNROLL_L
-    for (int l = 0; l <= N; l++) {
+    auto line_step = [&](int l) {
         T di = M.dt(l, i), dj = M.dt(l, j), dk = M.dt(l, k);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -410,8 +446,18 @@ __device__ __forceinline__ void node_gradient(int node, const MT& M, const T* s_
             get[q] = get[q] + dj * s_lift[q * n3p + Lay<N>::idx(i, l, k)];
             gze[q] = gze[q] + dk * s_lift[q * n3p + Lay<N>::idx(i, j, l)];
         }
+    };
+    if (LEAN) {
+#pragma unroll 1
+        for (int l = 0; l <= N; l++) line_step(l);
+    } else {
+#pragma unroll GALX_UNROLL_L
+        for (int l = 0; l <= N; l++) line_step(l);
     }
-    gradient_combine<N, T>(node, s_lflux, LHm, LHp, gm, gxi, get, gze, g);
+    if (LEAN)
+        gradient_combine_lean<N, T>(node, s_lflux, LHm, LHp, gm, gxi, get, gze, g);
+    else
+        gradient_combine<N, T>(node, s_lflux, LHm, LHp, gm, gxi, get, gze, g);
 }
 
 // 16x16x4 matrix-core step of type T: D = A B + C with A 16x4, B 4x16; lane l holds A[l % 16][l / 16] and
@@ -631,13 +677,15 @@ __global__ void __launch_bounds__(Dm<N>::block)
 // LDS: measured faster than face tasks that each evaluate their node's gradient (fewer, longer waves).
 // ---------------------------------------------------------------------------------------------------------
 // minimum waves per SIMD for k_lift: the mixed form's LDS (27 KB at N=7) lets four 512-thread workgroups share a
-// CU at 8 waves per SIMD (at most 64 registers, no spills): 2 % faster on MI355X at N=7.  FP64 stays at the
-// compiler's choice (120 registers): 6 waves fit the LDS but spill, and measured 37 % slower.  0 = compiler's choice
+// CU at 8 waves per SIMD (at most 64 registers, no spills): 2 % faster on MI355X at N=7.  FP64 at 6 waves: three
+// 44.5 KB workgroups per CU at N=7, at most 80 registers, which the lean gradient fits without spilling (74 at
+// N=7): FP64 k_lift 9 % faster than at the compiler's choice of 120 registers and two workgroups.  With the
+// plain gradient (88 bytes of spills per lane) it measured 63 % slower.  0 = compiler's choice
 #ifndef GALX_LIFT_MIXED_WAVES
 #define GALX_LIFT_MIXED_WAVES 8
 #endif
 #ifndef GALX_LIFT_FP64_WAVES
-#define GALX_LIFT_FP64_WAVES 0
+#define GALX_LIFT_FP64_WAVES 6
 #endif
 template <int N, typename T>
 struct LiftWaves {
@@ -685,7 +733,7 @@ __global__ void __launch_bounds__(Dm<N>::block, (LiftWaves<N, T>::value))
     T g[12];
     lift_face_fluxes<N, T, TT>(e, tid, mesh, utr, s_lift, s_lflux, gas, nullptr);
     __syncthreads();
-    if (gnode) node_gradient<N, T>(gn, M, s_lift, s_lflux, LHm, LHp, gm, g);
+    if (gnode) node_gradient<N, T, MatsD<N, T>, true>(gn, M, s_lift, s_lflux, LHm, LHp, gm, g);
     __syncthreads();
     // the stress tensor and heat flux of each surface node (the terms of EvalNormalDiffFlux3D, flux.f90:779-847,
     // before the normal) to its face slots: 9 values per face node instead of the 12 gradients (9 KB less LDS at
