@@ -474,8 +474,10 @@ __global__ void __launch_bounds__(Dm<N>::block, (AdjWaves<N>::value))
                         }
                     }
                 } else {
-                    // l = idx included like the forward: F#(U,U) has equal partial derivatives in its two
-                    // arguments, so the same formula gives 2 DVS(i,i) dF#/dU_i^T mu_i
+                    // l = idx included: F#(U,U) has equal partial derivatives in its two arguments, so the same
+                    // formula gives 2 DVS(i,i) dF#/dU_i^T mu_i.  The forward leaves this pair out; DVS(i,i) is a
+                    // rounding residue for Gauss-Lobatto, so the term is at rounding level either way, and the loop
+                    // over all l spills 8 to 16 bytes per lane less than one that skips it.
                     for (int l = 0; l <= N; l++) {
                         double P[6];
                         pair_bar(l, P);

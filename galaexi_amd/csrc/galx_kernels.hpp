@@ -1110,11 +1110,13 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
                         }
                     }
                 } else {
-                // all l, the diagonal included as in VolInt_splitForm_Kernel (volint.f90:307-317): DVolSurf(i,i)
-                // vanishes for Gauss-Lobatto up to rounding (dg.f90:229-233), and a uniform loop keeps the
-                // wave free of per-lane branches
+                // every partner l other than the node itself, l = idx + s (mod n) for s = 1..N: no per-lane
+                // branch.  VolInt_splitForm_Kernel (volint.f90:307-317) also adds the diagonal pair, whose
+                // DVolSurf(i,i) vanishes for Gauss-Lobatto up to rounding (dg.f90:229-233); the shared pairs above
+                // leave it out too (one flux of eight fewer on the k-lines at N=7: FP64 k_rhs 1.2 % faster)
 #pragma unroll GALX_UNROLL_SPLIT
-                for (int l = 0; l <= N; l++) {
+                for (int s = 1; s <= N; s++) {
+                    const int l = idx + s <= N ? idx + s : idx + s - n;
                     const int o = ln(l);
                     T Mo[3] = {s_m[o], s_m[n3p + o], s_m[2 * n3p + o]};
                     T G[5];
@@ -1149,18 +1151,24 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
         T Uo[5], fvo[4], Un[5], fvn[4];
         {
-            // from U at the face node, not from the own slot of the trace buffer: the same record, but that read
-            // goes to memory (measured slower on MI355X at N=7: 1.5 % in FP64, and 0.5 % in the mixed form, whose
-            // FP32 record is half the bytes of the FP64 state).  Formed by the node threads from the state they
-            // hold and passed through LDS instead, it measured 6 % slower in FP64 at N=7: the state then stays
-            // live through the volume terms and spills.
-            const double* uo = U + (long)e * 5 * n3 + node;
-            double c[5], tr[5];
+            // the record the neighbour reads.  Mixed form: from the own slot of the FP32 trace buffer, half the
+            // bytes of the FP64 state (0.5 % faster at N=7 on MI355X).  FP64: from U at the face node, which the
+            // load phase brought into the caches (the own slot measured 1.5 % slower at N=7).  Formed by the node
+            // threads from the state they hold and passed through LDS instead, it measured 6 % slower in FP64 at
+            // N=7: the state then stays live through the volume terms and spills.
+            if (sizeof(TT) == 4) {
+                const TT* ot = utr + ((side * 2 + (flip ? 1 : 0)) * 5) * n2 + pq;
 #pragma unroll
-            for (int v = 0; v < 5; v++) c[v] = uo[v * n3];
-            trace_record<sizeof(TT) == 4>(c, gd.kappa, gd.pref, tr);
+                for (int v = 0; v < 5; v++) Uo[v] = T(ot[v * n2]);
+            } else {
+                const double* uo = U + (long)e * 5 * n3 + node;
+                double c[5], tr[5];
 #pragma unroll
-            for (int v = 0; v < 5; v++) Uo[v] = T(TT(tr[v]));  // exactly the record the neighbour reads
+                for (int v = 0; v < 5; v++) c[v] = uo[v * n3];
+                trace_record<false>(c, gd.kappa, gd.pref, tr);
+#pragma unroll
+                for (int v = 0; v < 5; v++) Uo[v] = T(TT(tr[v]));
+            }
         }
         const TT* vo = vtr + ((side * 2 + (flip ? 1 : 0)) * 4) * n2 + pq;  // written by k_lift of this element
 #pragma unroll
