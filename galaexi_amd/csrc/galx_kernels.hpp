@@ -1151,9 +1151,10 @@ __global__ void __launch_bounds__(Dm<N>::block, (RhsWaves<N, RIEMANN, T>::value)
         int pq = mesh.inv[(flip * 6 + ls - 1) * n2 + ab];
         T Uo[5], fvo[4], Un[5], fvn[4];
         {
-            // the record the neighbour reads.  Mixed form: from the own slot of the FP32 trace buffer, half the
-            // bytes of the FP64 state (0.5 % faster at N=7 on MI355X).  FP64: from U at the face node, which the
-            // load phase brought into the caches (the own slot measured 1.5 % slower at N=7).  Formed by the node
+            // the record the neighbour reads.  Mixed form: from the own slot of the FP32 trace buffer (0.5 % faster
+            // at N=7 on MI355X, though it reads 15 B/DOF more from HBM than the gather of the FP64 state, which
+            // the load phase had brought into the caches).  FP64: from U at the face node (the own slot measured
+            // 1.5 % slower at N=7).  Formed by the node
             // threads from the state they hold and passed through LDS instead, it measured 6 % slower in FP64 at
             // N=7: the state then stays live through the volume terms and spills.
             if (sizeof(TT) == 4) {
