@@ -78,6 +78,34 @@ def roofline(kernel_name, alg_per_dof, nDOF, avg_s, traffic):
             "traffic_over_alg": (traffic / (alg_per_dof * nDOF)) if traffic else None}
 
 
+def stream_rates(nbytes, reps=10):
+    """STREAM-like rates on this GPU (GB/s of bytes read + written, HIP events on torch's stream, torch's own
+    kernels on float64 arrays of nbytes each): copy b = a, scale b = s a, add c = a + b, triad c = a + s b.  The best
+    of them is the achievable streaming rate to read the kernels' rooflines against, beside the 8 TB/s peak."""
+    import torch
+
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.full_like(a, 2.0)
+    c = torch.empty_like(a)
+    ops = {"copy": (lambda: b.copy_(a), 2), "scale": (lambda: torch.mul(a, 3.0, out=b), 2),
+           "add": (lambda: torch.add(a, b, out=c), 3), "triad": (lambda: torch.add(a, b, alpha=3.0, out=c), 3)}
+    rates = {}
+    for name, (f, arrays) in ops.items():
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        rates[name] = arrays * n * 8 * reps / (e0.elapsed_time(e1) / 1e3) / 1e9
+    del a, b, c
+    torch.cuda.empty_cache()
+    return rates
+
+
 def tgv_case(ne, N=7, noise=1e-3, curved=False):
     from galaexi_amd import mesh as M
     from galaexi_amd import testcases as T
@@ -359,6 +387,14 @@ def main(argv=None):
     }
     dt = r["dt"]
     del r
+    # the box's achievable streaming rate: STREAM-like kernels over arrays of the headline state's size (2.26 GB)
+    rates = stream_rates(5 * nDOF * 8)
+    cgbs = max(rates.values())
+    out["hbm_stream"] = {"GBs": cgbs, "by_kernel_GBs": rates, "array_bytes": 5 * nDOF * 8,
+                         "what": "best of copy, scale, add, triad (torch kernels, float64; bytes read + written over "
+                                 "the HIP-event time of 10 launches)"}
+    for key in ("roofline", "roofline_k_lift"):
+        out[key]["frac_of_stream"] = out[key]["achieved"] / cgbs
     if a.compare_steps:
         print(f"# compare: TGV N={N} {a.ne}^3 FP64 vs mixed, {a.compare_steps} steps", file=sys.stderr, flush=True)
         r64 = run_gpu(m, U, "fp64", a.compare_steps, 1, dt=dt)
@@ -384,6 +420,8 @@ def main(argv=None):
                                                      load_traffic("k_rhs_mixed", a.ne, N)),
                           "roofline_mixed_k_lift": roofline("k_lift<7,float,float>", mlift_b, nDOF, mavg["lift"],
                                                             load_traffic("k_lift_mixed", a.ne, N))}
+        for key in ("roofline_mixed", "roofline_mixed_k_lift"):
+            out["compare"][key]["frac_of_stream"] = out["compare"][key]["achieved"] / cgbs
         del r64, r32
     if a.cpu_steps:
         print(f"# cpu baseline: oracle, {a.cpu_steps} step(s) of the headline workload", file=sys.stderr, flush=True)
